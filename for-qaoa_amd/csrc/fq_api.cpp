@@ -1,0 +1,1093 @@
+// fq_api.cpp -- host side of libforqaoa: handles, validation, cost plans, the
+// layer schedules and the C ABI declared in include/forqaoa.h.
+#include "forqaoa.h"
+#include "fq_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace fq;
+
+// ---- errors ------------------------------------------------------------------
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define FQ_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err(FQ_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+extern "C" const char *fq_last_error(void) { return g_err; }
+extern "C" int fq_version(void) { return 100; }
+
+// ---- handles -------------------------------------------------------------------
+
+struct PlanEntry {
+    void *dmem = nullptr;
+    PlanDev dev{};
+};
+
+struct fq_graph {
+    int N = 0;
+    std::vector<double> W;  // N*N, W[min*N+max], current labels
+    std::map<std::string, PlanEntry> plans;
+};
+
+struct TimingRec {
+    int kind;
+    hipEvent_t e0, e1;
+    double bytes;
+};
+
+struct fq_state {
+    int N = 0, D = 0, B = 0, C = 0, dtype = 0;
+    size_t elem = 16, bytes = 0;
+    void *sv = nullptr;
+    hipStream_t own = nullptr, stream = nullptr;
+    int qmap[FQ_MAXN];
+    double init_amp = 0;
+    double *d_part = nullptr;  // expect partials
+    size_t part_cap = 0;
+    double *d_out2 = nullptr;
+    double *d_cand_p = nullptr;
+    uint64_t *d_cand_i = nullptr;
+    int cand_blocks = 0;
+    bool timing = false;
+    std::vector<TimingRec> pending;
+    std::vector<hipEvent_t> pool;
+    double t_ms[FQ_K_COUNT] = {0};
+    int64_t t_cnt[FQ_K_COUNT] = {0};
+    double t_bytes[FQ_K_COUNT] = {0};
+};
+
+// ---- timing ------------------------------------------------------------------
+
+static hipEvent_t pool_get(fq_state *st)
+{
+    if (!st->pool.empty()) {
+        hipEvent_t e = st->pool.back();
+        st->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+static int timing_collect(fq_state *st)
+{
+    for (auto &r : st->pending) {
+        FQ_HIP(hipEventSynchronize(r.e1));
+        float ms = 0;
+        FQ_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+        st->t_ms[r.kind] += ms;
+        st->t_cnt[r.kind] += 1;
+        st->t_bytes[r.kind] += r.bytes;
+        st->pool.push_back(r.e0);
+        st->pool.push_back(r.e1);
+    }
+    st->pending.clear();
+    return FQ_OK;
+}
+
+// Wrap one launch with events when timing is on.
+template <typename F>
+static int timed(fq_state *st, int kind, double bytes, F &&launch)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st->timing) {
+        if (st->pending.size() > 4096) {
+            int rc = timing_collect(st);
+            if (rc)
+                return rc;
+        }
+        e0 = pool_get(st);
+        e1 = pool_get(st);
+        if (!e0 || !e1)
+            return set_err(FQ_ERR_DEVICE, "hipEventCreate failed");
+        FQ_HIP(hipEventRecord(e0, st->stream));
+    }
+    hipError_t e = launch();
+    if (e != hipSuccess)
+        return set_err(FQ_ERR_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
+    if (st->timing) {
+        FQ_HIP(hipEventRecord(e1, st->stream));
+        st->pending.push_back({kind, e0, e1, bytes});
+    }
+    return FQ_OK;
+}
+
+// ---- geometry ------------------------------------------------------------------
+
+struct Geom {
+    int N, Q, a_log, nthr, nh;
+    uint8_t tpos[16];
+    uint8_t hpos[FQ_MAXN];
+};
+
+// tile: positions in order (thread bits first, the last a_log are register bits)
+static Geom make_geom(int N, const std::vector<int> &tile, int a_log)
+{
+    Geom g{};
+    g.N = N;
+    g.Q = (int)tile.size();
+    g.a_log = a_log;
+    g.nthr = g.Q - a_log;
+    g.nh = N - g.Q;
+    bool in_tile[FQ_MAXN] = {false};
+    for (int i = 0; i < g.Q; ++i) {
+        g.tpos[i] = (uint8_t)tile[i];
+        in_tile[tile[i]] = true;
+    }
+    int h = 0;
+    for (int p = 0; p < N; ++p)
+        if (!in_tile[p])
+            g.hpos[h++] = (uint8_t)p;
+    return g;
+}
+
+static void geom_to_args(const Geom &g, TileArgs &a)
+{
+    memset(&a, 0, sizeof(a));
+    a.N = g.N;
+    a.Q = g.Q;
+    a.nthr = g.nthr;
+    a.nh = g.nh;
+    memcpy(a.tpos, g.tpos, sizeof(a.tpos));
+    memcpy(a.hpos, g.hpos, sizeof(a.hpos));
+}
+
+static std::string geom_key(const Geom &g)
+{
+    std::string k;
+    k.push_back((char)g.N);
+    k.push_back((char)g.Q);
+    k.push_back((char)g.a_log);
+    k.append(reinterpret_cast<const char *>(g.tpos), g.Q);
+    return k;
+}
+
+// RX groups: gate bits among the thread bits, a_log per LDS phase.
+static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
+{
+    a.gate_mask = gate_mask;
+    std::vector<int> pend;
+    for (int i = g.nthr - 1; i >= 0; --i)
+        if ((gate_mask >> i) & 1u)
+            pend.push_back(i);
+    a.ngroups = 0;
+    size_t pos = 0;
+    while (pos < pend.size()) {
+        if (a.ngroups >= FQ_MAX_GROUPS)
+            return set_err(FQ_ERR_ARG, "too many LDS phases");
+        std::vector<int> bits;
+        for (int r = 0; r < g.a_log && pos < pend.size(); ++r)
+            bits.push_back(pend[pos++]);
+        const int real = (int)bits.size();
+        // fill the phase with tile bits that receive no gate in it
+        for (int i = 0; (int)bits.size() < g.a_log && i < g.Q; ++i)
+            if (std::find(bits.begin(), bits.end(), i) == bits.end())
+                bits.push_back(i);
+        std::vector<int> sorted = bits;
+        std::sort(sorted.begin(), sorted.end());
+        uint16_t gm = 0;
+        uint8_t gg = 0;
+        for (int r = 0; r < g.a_log; ++r) {
+            gm |= (uint16_t)(1u << sorted[r]);
+            for (int q = 0; q < real; ++q)
+                if (bits[q] == sorted[r])
+                    gg |= (uint8_t)(1u << r);
+        }
+        a.grp_mask[a.ngroups] = gm;
+        a.grp_gates[a.ngroups] = gg;
+        a.ngroups++;
+    }
+    return FQ_OK;
+}
+
+// ---- cost plans ------------------------------------------------------------------
+
+static uint64_t hash_bytes(const void *p, size_t n)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+static int get_plan(fq_graph *G, const Geom &g, PlanDev *out)
+{
+    const int N = G->N;
+    const uint64_t wh = hash_bytes(G->W.data(), G->W.size() * sizeof(double));
+    std::string key = geom_key(g);
+    key.append(reinterpret_cast<const char *>(&wh), sizeof(wh));
+    auto it = G->plans.find(key);
+    if (it != G->plans.end()) {
+        *out = it->second.dev;
+        return FQ_OK;
+    }
+    if (G->plans.size() >= 64) {
+        FQ_HIP(hipDeviceSynchronize());
+        for (auto &kv : G->plans)
+            hipFree(kv.second.dmem);
+        G->plans.clear();
+    }
+    int ti[FQ_MAXN], hi[FQ_MAXN];
+    for (int p = 0; p < N; ++p)
+        ti[p] = hi[p] = -1;
+    for (int i = 0; i < g.Q; ++i)
+        ti[g.tpos[i]] = i;
+    for (int h = 0; h < g.nh; ++h)
+        hi[g.hpos[h]] = h;
+
+    std::vector<std::vector<std::pair<int, double>>> csr(g.Q);
+    std::vector<double> cH(g.Q, 0.0), omega(std::max(g.nh, 1), 0.0);
+    std::vector<int> hh_u, hh_v;
+    std::vector<double> hh_w;
+    struct TE {
+        int a, b;
+        double w;
+    };
+    std::vector<TE> te;
+    double wtot = 0;
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j) {
+            const double w = G->W[(size_t)i * N + j];
+            if (w == 0)
+                continue;
+            wtot += w;
+            if (ti[i] < 0 && ti[j] < 0) {
+                hh_u.push_back(hi[i]);
+                hh_v.push_back(hi[j]);
+                hh_w.push_back(w);
+            } else if (ti[i] >= 0 && ti[j] < 0) {
+                csr[ti[i]].push_back({hi[j], w});
+                cH[ti[i]] += w;
+                omega[hi[j]] += w;
+            } else if (ti[i] < 0 && ti[j] >= 0) {
+                csr[ti[j]].push_back({hi[i], w});
+                cH[ti[j]] += w;
+                omega[hi[i]] += w;
+            } else {
+                te.push_back({ti[i], ti[j], w});
+            }
+        }
+    const int nthr = g.nthr, A = 1 << g.a_log, T = 1 << nthr;
+    std::vector<double> thr((size_t)(1 + g.a_log) * T, 0.0);
+    for (int t = 0; t < T; ++t) {
+        double base = 0;
+        for (const TE &e : te) {
+            const bool at = e.a < nthr, bt = e.b < nthr;
+            if (at && bt) {
+                base += e.w * (double)(((t >> e.a) ^ (t >> e.b)) & 1);
+            } else if (at != bt) {
+                const int th = at ? e.a : e.b;
+                const int r = (at ? e.b : e.a) - nthr;
+                const int z = (t >> th) & 1;
+                base += e.w * z;
+                thr[(size_t)(1 + r) * T + t] += e.w * (1 - 2 * z);
+            }
+        }
+        thr[t] = base;
+    }
+    PlanDev dev{};
+    for (int j = 0; j < 16; ++j)
+        dev.rr[j] = 0;
+    for (int j = 0; j < A; ++j) {
+        double v = 0;
+        for (const TE &e : te)
+            if (e.a >= nthr && e.b >= nthr)
+                v += e.w * (double)(((j >> (e.a - nthr)) ^ (j >> (e.b - nthr))) & 1);
+        dev.rr[j] = v;
+    }
+    dev.wtot = wtot;
+    dev.n_hh = (int)hh_w.size();
+
+    std::vector<int> off(g.Q + 1, 0);
+    std::vector<int> ch;
+    std::vector<double> cw;
+    for (int i = 0; i < g.Q; ++i) {
+        off[i] = (int)ch.size();
+        for (auto &pr : csr[i]) {
+            ch.push_back(pr.first);
+            cw.push_back(pr.second);
+        }
+    }
+    off[g.Q] = (int)ch.size();
+
+    // blob layout
+    size_t o_cH = 0;
+    size_t o_om = align16(o_cH + sizeof(double) * g.Q);
+    size_t o_cw = align16(o_om + sizeof(double) * omega.size());
+    size_t o_hw = align16(o_cw + sizeof(double) * std::max<size_t>(cw.size(), 1));
+    size_t o_thr = align16(o_hw + sizeof(double) * std::max<size_t>(hh_w.size(), 1));
+    size_t o_off = align16(o_thr + sizeof(double) * thr.size());
+    size_t o_ch = align16(o_off + sizeof(int) * off.size());
+    size_t o_hu = align16(o_ch + sizeof(int) * std::max<size_t>(ch.size(), 1));
+    size_t o_hv = align16(o_hu + sizeof(int) * std::max<size_t>(hh_u.size(), 1));
+    size_t total = align16(o_hv + sizeof(int) * std::max<size_t>(hh_v.size(), 1));
+    std::vector<char> blob(total, 0);
+    auto put = [&](size_t o, const void *src, size_t n) {
+        if (n)
+            memcpy(blob.data() + o, src, n);
+    };
+    put(o_cH, cH.data(), sizeof(double) * g.Q);
+    put(o_om, omega.data(), sizeof(double) * omega.size());
+    put(o_cw, cw.data(), sizeof(double) * cw.size());
+    put(o_hw, hh_w.data(), sizeof(double) * hh_w.size());
+    put(o_thr, thr.data(), sizeof(double) * thr.size());
+    put(o_off, off.data(), sizeof(int) * off.size());
+    put(o_ch, ch.data(), sizeof(int) * ch.size());
+    put(o_hu, hh_u.data(), sizeof(int) * hh_u.size());
+    put(o_hv, hh_v.data(), sizeof(int) * hh_v.size());
+    void *d = nullptr;
+    if (hipMalloc(&d, total) != hipSuccess)
+        return set_err(FQ_ERR_OOM, "plan allocation of %zu bytes failed", total);
+    if (hipMemcpy(d, blob.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return set_err(FQ_ERR_DEVICE, "plan upload failed");
+    }
+    char *b = (char *)d;
+    dev.cH = (const double *)(b + o_cH);
+    dev.omega = (const double *)(b + o_om);
+    dev.csr_w = (const double *)(b + o_cw);
+    dev.hh_w = (const double *)(b + o_hw);
+    dev.thr = (const double *)(b + o_thr);
+    dev.csr_off = (const int *)(b + o_off);
+    dev.csr_h = (const int *)(b + o_ch);
+    dev.hh_u = (const int *)(b + o_hu);
+    dev.hh_v = (const int *)(b + o_hv);
+    PlanEntry pe;
+    pe.dmem = d;
+    pe.dev = dev;
+    G->plans[key] = pe;
+    *out = dev;
+    return FQ_OK;
+}
+
+// ---- state --------------------------------------------------------------------------
+
+static int check_state(const fq_state *st)
+{
+    if (!st)
+        return set_err(FQ_ERR_ARG, "null state");
+    return FQ_OK;
+}
+
+static int tile_a_log(int dtype, int Q)
+{
+    const int want = dtype == FQ_C128 ? 3 : 4;
+    return std::max(1, std::min(want, Q));
+}
+
+extern "C" int fq_state_create(int N, int D, int B, int C, int dtype, fq_state **out)
+{
+    if (!out)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (dtype != FQ_C128 && dtype != FQ_C64)
+        return set_err(FQ_ERR_ARG, "dtype must be FQ_C128 or FQ_C64");
+    if (N < 1 || N > 36)
+        return set_err(FQ_ERR_ARG, "N=%d out of range [1, 36]", N);
+    if (D < 0 || D > N)
+        return set_err(FQ_ERR_ARG, "D=%d out of range", D);
+    if (D > 0)
+        return set_err(FQ_ERR_UNSUPPORTED,
+                       "D=%d: multi-device state-vector transposition needs 2^D GPUs; this build "
+                       "runs on one GPU (D=0)", D);
+    if (B < 0 || B > N - D)
+        return set_err(FQ_ERR_ARG, "B=%d must satisfy 0 <= B <= N-D=%d", B, N - D);
+    if (C < 1 || C > tile_max_q(dtype))
+        return set_err(FQ_ERR_ARG, "C=%d out of range [1, %d] for this dtype (2^C tile in LDS)", C,
+                       tile_max_q(dtype));
+    fq_state *st = new fq_state();
+    st->N = N;
+    st->D = D;
+    st->B = B;
+    st->C = C;
+    st->dtype = dtype;
+    st->elem = dtype == FQ_C128 ? 16 : 8;
+    st->bytes = ((size_t)1 << N) * st->elem;
+    for (int p = 0; p < FQ_MAXN; ++p)
+        st->qmap[p] = p;
+    st->init_amp = pow(1. / sqrt(2.), N);
+    if (hipMalloc(&st->sv, st->bytes) != hipSuccess) {
+        hipGetLastError();
+        delete st;
+        return set_err(FQ_ERR_OOM, "state allocation of %zu bytes failed", ((size_t)1 << N) * (dtype == FQ_C128 ? 16 : 8));
+    }
+    const int Qr = std::min(12, N);
+    st->part_cap = (size_t)1 << (N - Qr);
+    st->cand_blocks = (int)std::min<uint64_t>(1024, std::max<uint64_t>(1, ((uint64_t)1 << N) / 4096));
+    bool ok = hipMalloc(&st->d_part, sizeof(double) * 2 * st->part_cap) == hipSuccess &&
+              hipMalloc(&st->d_out2, sizeof(double) * 2) == hipSuccess &&
+              hipMalloc(&st->d_cand_p, sizeof(double) * FQ_TOPK_MAX * st->cand_blocks) == hipSuccess &&
+              hipMalloc(&st->d_cand_i, sizeof(uint64_t) * FQ_TOPK_MAX * st->cand_blocks) == hipSuccess &&
+              hipStreamCreateWithFlags(&st->own, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        hipGetLastError();
+        fq_state_destroy(st);
+        return set_err(FQ_ERR_OOM, "scratch allocation failed");
+    }
+    st->stream = st->own;
+    *out = st;
+    return FQ_OK;
+}
+
+extern "C" int fq_state_destroy(fq_state *st)
+{
+    if (!st)
+        return FQ_OK;
+    if (st->stream)
+        hipStreamSynchronize(st->stream);
+    for (auto &r : st->pending) {
+        hipEventDestroy(r.e0);
+        hipEventDestroy(r.e1);
+    }
+    for (auto e : st->pool)
+        hipEventDestroy(e);
+    hipFree(st->sv);
+    hipFree(st->d_part);
+    hipFree(st->d_out2);
+    hipFree(st->d_cand_p);
+    hipFree(st->d_cand_i);
+    if (st->own)
+        hipStreamDestroy(st->own);
+    delete st;
+    return FQ_OK;
+}
+
+extern "C" int fq_state_info(const fq_state *st, int *N, int *C, int *dtype, size_t *bytes)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (N)
+        *N = st->N;
+    if (C)
+        *C = st->C;
+    if (dtype)
+        *dtype = st->dtype;
+    if (bytes)
+        *bytes = st->bytes;
+    return FQ_OK;
+}
+
+extern "C" int fq_state_device_ptr(const fq_state *st, void **dptr)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (!dptr)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    *dptr = st->sv;
+    return FQ_OK;
+}
+
+extern "C" int fq_state_set_stream(fq_state *st, void *hip_stream)
+{
+    if (int rc = check_state(st))
+        return rc;
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    st->stream = hip_stream ? (hipStream_t)hip_stream : st->own;
+    return FQ_OK;
+}
+
+extern "C" int fq_state_init_zero(fq_state *st)
+{
+    if (int rc = check_state(st))
+        return rc;
+    for (int p = 0; p < FQ_MAXN; ++p)
+        st->qmap[p] = p;
+    return timed(st, FQ_K_COST, (double)st->bytes,
+                 [&] { return launch_fill_zero_one(st->dtype, st->sv, st->N, st->stream); });
+}
+
+extern "C" int fq_init_h(fq_state *st)
+{
+    if (int rc = check_state(st))
+        return rc;
+    st->init_amp = pow(1. / sqrt(2.), st->N);
+    return FQ_OK;
+}
+
+// ---- graph ------------------------------------------------------------------------
+
+extern "C" int fq_graph_create(int N, fq_graph **out)
+{
+    if (!out)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (N < 1 || N > FQ_MAXN)
+        return set_err(FQ_ERR_ARG, "N=%d out of range", N);
+    fq_graph *g = new fq_graph();
+    g->N = N;
+    g->W.assign((size_t)N * N, 0.0);
+    *out = g;
+    return FQ_OK;
+}
+
+extern "C" int fq_graph_add_edge(fq_graph *g, int u, int v, double w)
+{
+    if (!g)
+        return set_err(FQ_ERR_ARG, "null graph");
+    if (u < 0 || v < 0 || u >= g->N || v >= g->N)
+        return set_err(FQ_ERR_ARG, "edge (%d,%d) out of range for N=%d", u, v, g->N);
+    if (u == v)
+        return set_err(FQ_ERR_ARG, "self-loop (%d,%d) is not an edge of the cut", u, v);
+    if (!std::isfinite(w))
+        return set_err(FQ_ERR_ARG, "edge weight must be finite");
+    if (u > v)
+        std::swap(u, v);
+    g->W[(size_t)u * g->N + v] = w;
+    return FQ_OK;
+}
+
+extern "C" int fq_graph_destroy(fq_graph *g)
+{
+    if (!g)
+        return FQ_OK;
+    if (!g->plans.empty()) {
+        hipDeviceSynchronize();
+        for (auto &kv : g->plans)
+            hipFree(kv.second.dmem);
+    }
+    delete g;
+    return FQ_OK;
+}
+
+extern "C" int fq_graph_weights(const fq_graph *g, double *W)
+{
+    if (!g || !W)
+        return set_err(FQ_ERR_ARG, "null argument");
+    memcpy(W, g->W.data(), sizeof(double) * g->W.size());
+    return FQ_OK;
+}
+
+// mapping_after_swap_conti (src/swap.cu:27-50): exchange labels out+t, in+t.
+extern "C" int fq_graph_relabel(fq_graph *g, int swap_out, int swap_in, int num_swap)
+{
+    if (!g)
+        return set_err(FQ_ERR_ARG, "null graph");
+    const int N = g->N;
+    if (num_swap < 0 || swap_out < 0 || swap_out + num_swap > swap_in || swap_in + num_swap > N)
+        return set_err(FQ_ERR_ARG, "relabel blocks [%d,+%d) [%d,+%d) invalid for N=%d", swap_out,
+                       num_swap, swap_in, num_swap, N);
+    double *W = g->W.data();
+    for (int t = 0; t < num_swap; ++t) {
+        const int a = swap_out + t, b = swap_in + t;
+        for (int i = 0; i < N; ++i) {
+            if (i == a || i == b)
+                continue;
+            double &wa = i < a ? W[(size_t)i * N + a] : W[(size_t)a * N + i];
+            double &wb = i < b ? W[(size_t)i * N + b] : W[(size_t)b * N + i];
+            std::swap(wa, wb);
+        }
+    }
+    return FQ_OK;
+}
+
+// ---- gate ops -------------------------------------------------------------------------
+
+static int graph_matches(const fq_state *st, const fq_graph *g)
+{
+    if (!g)
+        return set_err(FQ_ERR_ARG, "null graph");
+    if (g->N != st->N)
+        return set_err(FQ_ERR_ARG, "graph has N=%d but the state has N=%d", g->N, st->N);
+    return FQ_OK;
+}
+
+static std::vector<int> low_tile(int Q)
+{
+    std::vector<int> t(Q);
+    for (int i = 0; i < Q; ++i)
+        t[i] = i;
+    return t;
+}
+
+// Cost-phase launch arguments: angle, launch-control amplitude and the
+// register-register phase factors exp(i*gamma*rr[j]).
+static void set_cost(TileArgs &a, double gamma, double init_amp)
+{
+    a.gamma = gamma;
+    a.init_amp = init_amp;
+    a.has_rr = 0;
+    for (int j = 0; j < 16; ++j) {
+        if (a.plan.rr[j] != 0)
+            a.has_rr = 1;
+        double sn, cs;
+        sincos(gamma * a.plan.rr[j], &sn, &cs);
+        a.rr_c[j] = cs;
+        a.rr_s[j] = sn;
+    }
+}
+
+// Cost phase alone: 256 threads x 16 amplitudes per work-group, no LDS.
+static int cost_pass(fq_state *st, fq_graph *g, double gamma, bool first)
+{
+    const int Q = std::min(12, st->N);
+    const int a_log = std::max(1, std::min(4, Q));
+    Geom geom = make_geom(st->N, low_tile(Q), a_log);
+    TileArgs a;
+    geom_to_args(geom, a);
+    if (int rc = get_plan(g, geom, &a.plan))
+        return rc;
+    set_cost(a, gamma, st->init_amp);
+    const double bytes = (first ? 1.0 : 2.0) * (double)st->bytes;
+    return timed(st, FQ_K_COST, bytes, [&] {
+        return launch_tile(st->dtype, a_log, true, first, false, st->sv, a, st->stream);
+    });
+}
+
+// One tile pass with RX on gate positions (and optionally the cost phase).
+static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vector<int> &gates,
+                     double beta, fq_graph *g, double gamma, bool cost, bool first, int kind)
+{
+    const int Q = (int)tile.size();
+    const int a_log = tile_a_log(st->dtype, Q);
+    Geom geom = make_geom(st->N, tile, a_log);
+    TileArgs a;
+    geom_to_args(geom, a);
+    uint32_t gm = 0;
+    for (int p : gates) {
+        int ti = -1;
+        for (int i = 0; i < Q; ++i)
+            if (tile[i] == p)
+                ti = i;
+        if (ti < 0)
+            return set_err(FQ_ERR_ARG, "internal: gate %d not in tile", p);
+        gm |= 1u << ti;
+    }
+    if (int rc = set_gates(a, geom, gm))
+        return rc;
+    double s, c;
+    sincos(beta / 2, &s, &c);
+    a.rx_c = c;
+    a.rx_s = s;
+    if (cost) {
+        if (int rc = get_plan(g, geom, &a.plan))
+            return rc;
+        set_cost(a, gamma, st->init_amp);
+    }
+    const double bytes = (first ? 1.0 : 2.0) * (double)st->bytes;
+    return timed(st, kind, bytes, [&] {
+        return launch_tile(st->dtype, a_log, cost, first, true, st->sv, a, st->stream);
+    });
+}
+
+static int carriers_for(int dtype) { return dtype == FQ_C128 ? 3 : 4; }
+
+// Split arbitrary RX targets into tile passes (targets below C: one chunk pass).
+static int rx_targets(fq_state *st, std::vector<int> targets, double beta)
+{
+    const int N = st->N;
+    const int Ceff = std::min(st->C, N);
+    std::sort(targets.begin(), targets.end());
+    if (targets.empty())
+        return FQ_OK;
+    if (targets.back() < Ceff)
+        return tile_pass(st, low_tile(Ceff), targets, beta, nullptr, 0, false, false, FQ_K_RX);
+    const int Qmax = std::min(tile_max_q(st->dtype), N);
+    const int per = std::max(1, Qmax - std::min(carriers_for(st->dtype), Qmax - 1));
+    size_t pos = 0;
+    while (pos < targets.size()) {
+        std::vector<int> chunk;
+        while (pos < targets.size() && (int)chunk.size() < per)
+            chunk.push_back(targets[pos++]);
+        std::vector<int> tile = chunk;
+        for (int p = 0; (int)tile.size() < Qmax && p < N; ++p)
+            if (std::find(chunk.begin(), chunk.end(), p) == chunk.end())
+                tile.push_back(p);
+        std::sort(tile.begin(), tile.end());
+        if (int rc = tile_pass(st, tile, chunk, beta, nullptr, 0, false, false, FQ_K_RX))
+            return rc;
+    }
+    return FQ_OK;
+}
+
+extern "C" int fq_cost_layer(fq_state *st, fq_graph *g, double gamma, int is_first)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (int rc = graph_matches(st, g))
+        return rc;
+    if (!std::isfinite(gamma))
+        return set_err(FQ_ERR_ARG, "gamma must be finite");
+    return cost_pass(st, g, gamma, is_first != 0);
+}
+
+extern "C" int fq_multi_rx(fq_state *st, const int *targets, int gate_size, double beta)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (gate_size < 0 || (gate_size > 0 && !targets))
+        return set_err(FQ_ERR_ARG, "bad target list");
+    if (!std::isfinite(beta))
+        return set_err(FQ_ERR_ARG, "beta must be finite");
+    std::vector<int> t(targets, targets + gate_size);
+    std::vector<bool> seen(st->N, false);
+    for (int q : t) {
+        if (q < 0 || q >= st->N)
+            return set_err(FQ_ERR_ARG, "target %d out of range for N=%d", q, st->N);
+        if (seen[q])
+            return set_err(FQ_ERR_ARG, "target %d listed twice", q);
+        seen[q] = true;
+    }
+    return rx_targets(st, t, beta);
+}
+
+extern "C" int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_swap)
+{
+    if (int rc = check_state(st))
+        return rc;
+    const int N = st->N, k = num_swap;
+    if (k < 0 || swap_out < 0 || swap_out + k > swap_in || swap_in + k > N)
+        return set_err(FQ_ERR_ARG, "swap blocks [%d,+%d) [%d,+%d) invalid for N=%d", swap_out, k,
+                       swap_in, k, N);
+    if (k == 0)
+        return FQ_OK;
+    SwapArgs a;
+    memset(&a, 0, sizeof(a));
+    a.N = N;
+    a.out = swap_out;
+    a.in = swap_in;
+    a.k = k;
+    bool used[FQ_MAXN] = {false};
+    for (int i = 0; i < k; ++i)
+        used[swap_out + i] = used[swap_in + i] = true;
+    for (int p = 0; p < N; ++p)
+        if (!used[p])
+            a.rpos[a.nr++] = (uint8_t)p;
+    // out == 0 (the driver's case): coalesced LDS-tiled transpose; otherwise
+    // (or for tiny blocks) one thread per pair.
+    if (swap_out == 0 && k >= 3) {
+        a.t = std::min(k, 5);
+        const uint64_t m = 1ull << (k - a.t);
+        a.npairs = m * (m + 1) / 2;
+    } else {
+        a.t = -1;
+    }
+    const double bytes = 2.0 * (double)st->bytes * (1.0 - std::ldexp(1.0, -k));
+    int rc = timed(st, FQ_K_SWAP, bytes, [&] { return launch_swap(st->dtype, st->sv, a, st->stream); });
+    if (rc)
+        return rc;
+    for (int i = 0; i < k; ++i)
+        std::swap(st->qmap[swap_out + i], st->qmap[swap_in + i]);
+    return FQ_OK;
+}
+
+// ---- schedules -----------------------------------------------------------------------
+
+struct FusedPass {
+    std::vector<int> tile, gates;
+};
+
+static std::vector<FusedPass> fused_passes(int N, int dtype)
+{
+    std::vector<FusedPass> ps;
+    const int Qmax = std::min(tile_max_q(dtype), N);
+    FusedPass p1;
+    p1.tile = low_tile(Qmax);
+    p1.gates = p1.tile;
+    ps.push_back(p1);
+    int next = Qmax;
+    const int cmin = std::min(carriers_for(dtype), Qmax - 1);
+    while (next < N) {
+        const int n_new = std::min(Qmax - cmin, N - next);
+        const int c = std::min(Qmax - n_new, next);
+        FusedPass p;
+        for (int i = 0; i < c; ++i)
+            p.tile.push_back(i);
+        for (int i = 0; i < n_new; ++i) {
+            p.tile.push_back(next + i);
+            p.gates.push_back(next + i);
+        }
+        next += n_new;
+        ps.push_back(p);
+    }
+    return ps;
+}
+
+extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
+                      int schedule)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (int rc = graph_matches(st, g))
+        return rc;
+    if (P < 0 || (P > 0 && (!gammas || !betas)))
+        return set_err(FQ_ERR_ARG, "bad layer count or angle arrays");
+    for (int p = 0; p < P; ++p)
+        if (!std::isfinite(gammas[p]) || !std::isfinite(betas[p]))
+            return set_err(FQ_ERR_ARG, "angles must be finite");
+    const int N = st->N;
+    if (schedule == FQ_SCHED_REFERENCE) {
+        // src/Weighted.cpp:97-137
+        std::vector<int> sizes;
+        for (int i = 0; i < N; i += st->C)
+            sizes.push_back(std::min(st->C, N - i));
+        for (int p = 0; p < P; ++p) {
+            if (int rc = cost_pass(st, g, gammas[p], p == 0))
+                return rc;
+            int swap_in = 0;
+            for (size_t i = 0; i < sizes.size(); ++i) {
+                std::vector<int> t(sizes[i]);
+                for (int q = 0; q < sizes[i]; ++q)
+                    t[q] = q;
+                if (int rc = rx_targets(st, t, betas[p]))
+                    return rc;
+                if (i + 1 < sizes.size()) {
+                    swap_in += sizes[i];
+                    if (int rc = fq_swap_conti(st, 0, swap_in, sizes[i + 1]))
+                        return rc;
+                    if (int rc = fq_graph_relabel(g, 0, swap_in, sizes[i + 1]))
+                        return rc;
+                }
+            }
+        }
+        return FQ_OK;
+    }
+    if (schedule != FQ_SCHED_FUSED)
+        return set_err(FQ_ERR_ARG, "unknown schedule %d", schedule);
+    const std::vector<FusedPass> ps = fused_passes(N, st->dtype);
+    for (int p = 0; p < P; ++p) {
+        for (size_t i = 0; i < ps.size(); ++i) {
+            const bool cost = i == 0;
+            if (int rc = tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, gammas[p], cost,
+                                   cost && p == 0, FQ_K_FUSED))
+                return rc;
+        }
+    }
+    return FQ_OK;
+}
+
+// ---- observables -------------------------------------------------------------------------
+
+extern "C" int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *norm2)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (int rc = graph_matches(st, g))
+        return rc;
+    if (!expect)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    const int Q = std::min(12, st->N);
+    const int a_log = std::max(1, std::min(4, Q));
+    Geom geom = make_geom(st->N, low_tile(Q), a_log);
+    TileArgs a;
+    geom_to_args(geom, a);
+    if (int rc = get_plan(g, geom, &a.plan))
+        return rc;
+    int rc = timed(st, FQ_K_REDUCE, (double)st->bytes, [&] {
+        return launch_expect(st->dtype, a_log, st->sv, a, st->d_part, st->d_out2, st->stream);
+    });
+    if (rc)
+        return rc;
+    double h[2];
+    FQ_HIP(hipMemcpyAsync(h, st->d_out2, sizeof(h), hipMemcpyDeviceToHost, st->stream));
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    *expect = h[0];
+    if (norm2)
+        *norm2 = h[1];
+    return FQ_OK;
+}
+
+static uint64_t to_logical_index(const fq_state *st, uint64_t b)
+{
+    uint64_t L = 0;
+    for (int p = 0; p < st->N; ++p)
+        L |= ((b >> p) & 1ull) << st->qmap[p];
+    return L;
+}
+
+static uint64_t to_physical_index(const fq_state *st, uint64_t L)
+{
+    uint64_t b = 0;
+    for (int p = 0; p < st->N; ++p)
+        b |= ((L >> st->qmap[p]) & 1ull) << p;
+    return b;
+}
+
+extern "C" int fq_topk(fq_state *st, int k, uint64_t *idx, double *prob)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (k < 1 || k > FQ_TOPK_MAX || !idx || !prob)
+        return set_err(FQ_ERR_ARG, "k=%d must be in [1, %d] with non-null outputs", k, FQ_TOPK_MAX);
+    if ((uint64_t)k > (1ull << st->N))
+        return set_err(FQ_ERR_ARG, "k=%d exceeds the number of basis states", k);
+    const int nb = st->cand_blocks;
+    int rc = timed(st, FQ_K_TOPK, (double)st->bytes, [&] {
+        return launch_topk(st->dtype, st->sv, st->N, nb, st->d_cand_p, st->d_cand_i, st->stream);
+    });
+    if (rc)
+        return rc;
+    std::vector<double> cp((size_t)nb * FQ_TOPK_MAX);
+    std::vector<uint64_t> ci((size_t)nb * FQ_TOPK_MAX);
+    FQ_HIP(hipMemcpyAsync(cp.data(), st->d_cand_p, sizeof(double) * cp.size(),
+                          hipMemcpyDeviceToHost, st->stream));
+    FQ_HIP(hipMemcpyAsync(ci.data(), st->d_cand_i, sizeof(uint64_t) * ci.size(),
+                          hipMemcpyDeviceToHost, st->stream));
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    std::vector<size_t> order(cp.size());
+    for (size_t i = 0; i < order.size(); ++i)
+        order[i] = i;
+    std::partial_sort(order.begin(), order.begin() + k, order.end(), [&](size_t x, size_t y) {
+        return cp[x] > cp[y] || (cp[x] == cp[y] && ci[x] < ci[y]);
+    });
+    for (int i = 0; i < k; ++i) {
+        prob[i] = cp[order[i]];
+        idx[i] = to_logical_index(st, ci[order[i]]);
+    }
+    return FQ_OK;
+}
+
+extern "C" int fq_qubit_map(const fq_state *st, int *map)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (!map)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    for (int p = 0; p < st->N; ++p)
+        map[p] = st->qmap[p];
+    return FQ_OK;
+}
+
+extern "C" int fq_state_copy_out(fq_state *st, void *host, size_t bytes, int logical)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (!host || bytes != st->bytes)
+        return set_err(FQ_ERR_ARG, "copy_out needs exactly %zu bytes", st->bytes);
+    bool ident = true;
+    for (int p = 0; p < st->N; ++p)
+        ident = ident && st->qmap[p] == p;
+    if (!logical || ident) {
+        FQ_HIP(hipMemcpyAsync(host, st->sv, bytes, hipMemcpyDeviceToHost, st->stream));
+        FQ_HIP(hipStreamSynchronize(st->stream));
+        return FQ_OK;
+    }
+    std::vector<char> tmp(bytes);
+    FQ_HIP(hipMemcpyAsync(tmp.data(), st->sv, bytes, hipMemcpyDeviceToHost, st->stream));
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    const uint64_t n = 1ull << st->N;
+    char *dst = (char *)host;
+    for (uint64_t b = 0; b < n; ++b)
+        memcpy(dst + to_logical_index(st, b) * st->elem, tmp.data() + b * st->elem, st->elem);
+    return FQ_OK;
+}
+
+extern "C" int fq_state_copy_in(fq_state *st, const void *host, size_t bytes)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (!host || bytes != st->bytes)
+        return set_err(FQ_ERR_ARG, "copy_in needs exactly %zu bytes", st->bytes);
+    FQ_HIP(hipMemcpyAsync(st->sv, host, bytes, hipMemcpyHostToDevice, st->stream));
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    return FQ_OK;
+}
+
+extern "C" int fq_state_gather(fq_state *st, const uint64_t *idx, size_t n, double *out, int logical)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (n == 0)
+        return FQ_OK;
+    if (!idx || !out)
+        return set_err(FQ_ERR_ARG, "null argument");
+    const uint64_t lim = 1ull << st->N;
+    std::vector<uint64_t> phys(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (idx[i] >= lim)
+            return set_err(FQ_ERR_ARG, "index %llu out of range", (unsigned long long)idx[i]);
+        phys[i] = logical ? to_physical_index(st, idx[i]) : idx[i];
+    }
+    uint64_t *d_idx = nullptr;
+    double *d_out = nullptr;
+    if (hipMalloc(&d_idx, sizeof(uint64_t) * n) != hipSuccess ||
+        hipMalloc(&d_out, sizeof(double) * 2 * n) != hipSuccess) {
+        hipFree(d_idx);
+        return set_err(FQ_ERR_OOM, "gather scratch allocation failed");
+    }
+    int rc = FQ_OK;
+    if (hipMemcpyAsync(d_idx, phys.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, st->stream) != hipSuccess ||
+        launch_gather(st->dtype, st->sv, d_idx, n, d_out, st->stream) != hipSuccess ||
+        hipMemcpyAsync(out, d_out, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st->stream) != hipSuccess ||
+        hipStreamSynchronize(st->stream) != hipSuccess)
+        rc = set_err(FQ_ERR_DEVICE, "gather failed");
+    hipFree(d_idx);
+    hipFree(d_out);
+    return rc;
+}
+
+extern "C" int fq_sync(fq_state *st)
+{
+    if (int rc = check_state(st))
+        return rc;
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    return FQ_OK;
+}
+
+extern "C" int fq_timing_enable(fq_state *st, int on)
+{
+    if (int rc = check_state(st))
+        return rc;
+    st->timing = on != 0;
+    return FQ_OK;
+}
+
+extern "C" int fq_timing_get(fq_state *st, int kind, double *ms, int64_t *count, double *bytes)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (kind < 0 || kind >= FQ_K_COUNT)
+        return set_err(FQ_ERR_ARG, "unknown kernel class %d", kind);
+    if (int rc = timing_collect(st))
+        return rc;
+    if (ms)
+        *ms = st->t_ms[kind];
+    if (count)
+        *count = st->t_cnt[kind];
+    if (bytes)
+        *bytes = st->t_bytes[kind];
+    return FQ_OK;
+}
+
+extern "C" int fq_timing_reset(fq_state *st)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (int rc = timing_collect(st))
+        return rc;
+    for (int k = 0; k < FQ_K_COUNT; ++k) {
+        st->t_ms[k] = 0;
+        st->t_cnt[k] = 0;
+        st->t_bytes[k] = 0;
+    }
+    return FQ_OK;
+}

@@ -1,0 +1,85 @@
+// fq_internal.h -- structures shared by the gfx950 kernels and the host side of
+// libforqaoa.  Not part of the public ABI (include/forqaoa.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstddef>
+
+#define FQ_MAXN 40         // qubits (2^40 complex64 = 8 TiB: the HBM limit bites first)
+#define FQ_MAXQ 14         // tile qubits held in LDS (2^14 complex64 = 128 KiB)
+#define FQ_MAX_GROUPS 16   // LDS re-mapping phases per tile pass
+#define FQ_TOPK_MAX 16
+
+namespace fq {
+
+// Device view of a cost plan: the Max-Cut quadratic form split for one tile
+// geometry.  For a tile (2^Q amplitudes whose index bits at tile positions
+// vary) with the other N-Q "high" bits fixed by the tile index:
+//   Cut(z) = K(zH) + sum_{tile bit i} c_i(zH) z_i + Cut_tile(z_tile)
+//   c_i(zH) = cH[i] - 2 sum_{(h,w) in csr(i)} w zH_h
+//   K(zH)  = sum_{hh edges} w [zH_u != zH_v] + sum_h omega[h] zH_h
+// The tile bits split into nthr thread bits and A_LOG register bits; the
+// thread-dependent, tile-independent part comes from per-thread tables
+//   thr[t]                   = edges inside the thread bits + register-thread
+//                              edges' constant parts
+//   thr[(1+r)*2^nthr + t]    = d_r(t) (coefficient of register bit r)
+// and the register-register edges from rr[j] (j = register index).
+struct PlanDev {
+    const double *cH;      // [Q]
+    const int *csr_off;    // [Q+1]
+    const int *csr_h;      // [nnz] high-bit index
+    const double *csr_w;   // [nnz]
+    const int *hh_u;       // [n_hh] high-bit index
+    const int *hh_v;       // [n_hh]
+    const double *hh_w;    // [n_hh]
+    const double *omega;   // [nh]
+    const double *thr;     // [(1 + A_LOG) << nthr]
+    int n_hh;
+    int pad_;
+    double wtot;           // sum of all weights
+    double rr[16];         // register-register cut contributions
+};
+
+// One tile pass: load 2^Q amplitudes per work-group (tile bits tpos[], high
+// bits hpos[]), optionally apply the cost phase, RX on the gate bits, store.
+struct TileArgs {
+    int N, Q, nthr, nh;
+    uint8_t tpos[16];
+    uint8_t hpos[FQ_MAXN];
+    uint32_t gate_mask;      // over tile bits
+    int ngroups;
+    uint16_t grp_mask[FQ_MAX_GROUPS];  // tile-bit mask of each LDS phase (A_LOG bits)
+    uint8_t grp_gates[FQ_MAX_GROUPS];  // register bits of that phase that receive RX
+    double rx_c, rx_s;       // cos(beta/2), sin(beta/2) (host-computed, src/rx.cu:121)
+    double gamma, init_amp;  // cost phase angle, launch-control amplitude
+    int has_rr;              // any register-register edge in the plan
+    double rr_c[16], rr_s[16];  // cos/sin(gamma * plan.rr[j]), host-evaluated
+    PlanDev plan;
+};
+
+struct SwapArgs {
+    int N, out, in, k;
+    int t;                   // tiled: sub-tile log2 side
+    int nr;                  // number of rest bits
+    uint8_t rpos[FQ_MAXN];   // rest bit -> physical position
+    uint64_t npairs;         // tiled: number of (X <= Y) sub-tile pairs
+};
+
+// Launchers (fq_kernels.hip).  All enqueue on `stream` and return the HIP
+// launch status.
+hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, void *sv,
+                       const TileArgs &a, hipStream_t stream);
+hipError_t launch_swap(int dtype, void *sv, const SwapArgs &a, hipStream_t stream);
+hipError_t launch_expect(int dtype, int a_log, const void *sv, const TileArgs &a, double *partials,
+                         double *out2, hipStream_t stream);
+hipError_t launch_topk(int dtype, const void *sv, int N, int nblocks, double *cand_p,
+                       uint64_t *cand_i, hipStream_t stream);
+hipError_t launch_gather(int dtype, const void *sv, const uint64_t *idx, size_t n, double *out,
+                         hipStream_t stream);
+hipError_t launch_fill_zero_one(int dtype, void *sv, int N, hipStream_t stream);
+int tile_max_q(int dtype);
+size_t tile_lds_bytes(int dtype, int Q);
+
+}  // namespace fq

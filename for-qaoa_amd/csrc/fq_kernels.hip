@@ -1,0 +1,723 @@
+// fq_kernels.hip -- gfx950 kernels of the QAOA Max-Cut layer loop.
+//
+// Everything on this path is HBM streaming: each kernel reads and writes every
+// amplitude once per launch (the block transposition moves (1 - 2^-k) of
+// them), so the kernels are shaped for coalesced 64-lane access, LDS staging
+// and few VALU operations per byte.  No MFMA: there is no dense contraction.
+//
+//  k_tile    one "tile pass": a work-group owns 2^Q amplitudes whose index
+//            bits at the Q tile positions vary (the high bits are fixed by
+//            the tile index).  Optional cost phase (Max-Cut value per basis
+//            index from a per-tile / per-thread decomposition of the
+//            quadratic form, then exp(i*gamma*s/2)), optional launch-control
+//            write (no read), optional RX on any subset of the tile bits.
+//            Register bits are butterflied in registers; the other gate bits
+//            are remapped into registers through padded LDS phases.
+//            Replaces _rotationCompressionWeighted_conti (src/rzz.cu:121-156)
+//            and _multirotateX_unroll (src/rx.cu:56-87).
+//  k_swap_tiled / k_swap_generic
+//            in-place qubit-block exchange [out,out+k) <-> [in,in+k);
+//            replaces _swap_gate_conti (src/swap.cu:76-102) without the
+//            host-built pair table: 2^t x 2^t sub-tiles are transposed
+//            through LDS so both sides are read and written coalesced.
+//  k_expect / k_finalize   <C> and sum|psi|^2, FP64, fixed reduction order.
+//  k_topk    per-block top-16 candidates (bitonic sort in LDS).
+#include "fq_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace fq {
+
+template <typename R> struct CT;
+template <> struct CT<double> { typedef double2 T; };
+template <> struct CT<float> { typedef float2 T; };
+
+__device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
+
+// RX(beta) butterfly, the matrix [[c, -i s], [-i s, c]] (src/rx.cu:46-49).
+template <typename T, typename R>
+__device__ __forceinline__ void rx_pair(T &q0, T &q1, R c, R s)
+{
+    T o0, o1;
+    o0.x = q0.x * c + q1.y * s;
+    o0.y = q0.y * c - q1.x * s;
+    o1.x = q0.y * s + q1.x * c;
+    o1.y = -q0.x * s + q1.y * c;
+    q0 = o0;
+    q1 = o1;
+}
+
+template <typename T, typename R, int A_LOG>
+__device__ __forceinline__ void rx_registers(T *v, unsigned mask, R c, R s)
+{
+    constexpr int A = 1 << A_LOG;
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r) {
+        if ((mask >> r) & 1u) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                if (!((j >> r) & 1))
+                    rx_pair<T, R>(v[j], v[j | (1 << r)], c, s);
+        }
+    }
+}
+
+// v *= (c + i s)
+template <typename T, typename R>
+__device__ __forceinline__ void cmul_by(T &v, R c, R s)
+{
+    const R x = v.x, y = v.y;
+    v.x = x * c - y * s;
+    v.y = x * s + y * c;
+}
+
+// Per-tile part of the cut decomposition: c_i(zH) for every tile bit and the
+// partial sums of K(zH) (summed by the caller after a barrier).
+__device__ __forceinline__ void cut_tile_prologue(const TileArgs &a, uint64_t tile, double *s_c,
+                                                  double *s_red, int t, int nthreads)
+{
+    const PlanDev &p = a.plan;
+    for (int i = t; i < a.Q; i += nthreads) {
+        double acc = 0.0;
+        for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
+            acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
+        s_c[i] = p.cH[i] - 2.0 * acc;
+    }
+    const int lanes = nthreads < 64 ? nthreads : 64;
+    if (t < lanes) {
+        double acc = 0.0;
+        for (int e = t; e < p.n_hh; e += lanes)
+            acc += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
+        for (int h = t; h < a.nh; h += lanes)
+            acc += p.omega[h] * (double)((tile >> h) & 1ull);
+        s_red[t] = acc;
+    }
+}
+
+// Per-thread part: base cut of the thread's amplitudes and the coefficient of
+// each register bit.  Call after the barrier that publishes s_c / s_red.
+template <int A_LOG>
+__device__ __forceinline__ double cut_thread_setup(const TileArgs &a, const double *s_c,
+                                                   const double *s_red, int t, int nthreads,
+                                                   double *cr)
+{
+    const PlanDev &p = a.plan;
+    const int lanes = nthreads < 64 ? nthreads : 64;
+    double K = 0.0;
+    for (int l = 0; l < lanes; ++l)
+        K += s_red[l];
+    double base = K + p.thr[t];
+    for (int i = 0; i < a.nthr; ++i)
+        base += s_c[i] * (double)((t >> i) & 1);
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r)
+        cr[r] = s_c[a.nthr + r] + p.thr[((1 + r) << a.nthr) + t];
+    return base;
+}
+
+// Cut of register j (j is a compile-time constant after unrolling).
+template <int A_LOG>
+__device__ __forceinline__ double cut_of(const TileArgs &a, double base, const double *cr, int j)
+{
+    double cut = base + a.plan.rr[j];
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r)
+        if ((j >> r) & 1)
+            cut += cr[r];
+    return cut;
+}
+
+// exp(i*theta) with theta = s*gamma/2 as src/rzz.cu:146 (FP64 sincos).
+__device__ __forceinline__ void phase_of(double theta, double &c, double &s)
+{
+    sincos(theta, &s, &c);
+}
+// complex64: range-reduce theta in FP64, evaluate in FP32.
+__device__ __forceinline__ void phase_of(double theta, float &c, float &s)
+{
+    const double two_pi_hi = 6.283185307179586232;
+    const double two_pi_lo = 2.4492935982947064e-16;
+    const double n = rint(theta * 0.15915494309189535);
+    double r = fma(-n, two_pi_hi, theta);
+    r = fma(-n, two_pi_lo, r);
+    sincosf((float)r, &s, &c);
+}
+
+template <typename R, int A_LOG, bool COST, bool FIRST, bool GATES>
+__global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a)
+{
+    typedef typename CT<R>::T T;
+    constexpr int A = 1 << A_LOG;
+    extern __shared__ __align__(16) unsigned char fq_smem[];
+    __shared__ double s_c[FQ_MAXQ + 2];
+    __shared__ double s_red[64];
+
+    const int t = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const uint64_t tile = blockIdx.x;
+
+    uint64_t base = 0;
+    for (int i = 0; i < a.nh; ++i)
+        base |= ((tile >> i) & 1ull) << a.hpos[i];
+    uint64_t toff = 0;
+    for (int i = 0; i < a.nthr; ++i)
+        toff |= (uint64_t)((t >> i) & 1) << a.tpos[i];
+    uint64_t roff[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                o |= 1ull << a.tpos[a.nthr + r];
+        roff[j] = o;
+    }
+    T *const p0 = sv + (base + toff);
+
+    T v[A];
+    if (!FIRST) {
+#pragma unroll
+        for (int j = 0; j < A; ++j)
+            v[j] = p0[roff[j]];
+    }
+
+    if (COST) {
+        // exp(i*gamma*s/2) with s = 2*Cut - W, applied as a product of factors:
+        // the thread's base angle, one factor per register bit set in j, and
+        // the register-register term (host-evaluated per launch).
+        cut_tile_prologue(a, tile, s_c, s_red, t, nthreads);
+        __syncthreads();
+        double cr[A_LOG > 0 ? A_LOG : 1];
+        const double cbase = cut_thread_setup<A_LOG>(a, s_c, s_red, t, nthreads, cr);
+        const double theta0 = (2.0 * cbase - a.plan.wtot) * a.gamma / 2;
+        R c0, s0;
+        phase_of(theta0, c0, s0);
+        if (FIRST) {
+            const R amp = (R)a.init_amp;
+            c0 *= amp;
+            s0 *= amp;
+        }
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+            if (FIRST) {
+                v[j].x = c0;
+                v[j].y = s0;
+            } else {
+                cmul_by(v[j], c0, s0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r) {
+            R cf, sf;
+            phase_of(a.gamma * cr[r], cf, sf);
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                if ((j >> r) & 1)
+                    cmul_by(v[j], cf, sf);
+        }
+        if (a.has_rr) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                cmul_by(v[j], (R)a.rr_c[j], (R)a.rr_s[j]);
+        }
+    }
+
+    if (GATES) {
+        const R c = (R)a.rx_c, s = (R)a.rx_s;
+        rx_registers<T, R, A_LOG>(v, a.gate_mask >> a.nthr, c, s);
+        if (a.ngroups > 0) {
+            T *lds = reinterpret_cast<T *>(fq_smem);
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                lds[lds_pad(t | (j << a.nthr))] = v[j];
+            __syncthreads();
+            const int qmask = (1 << a.Q) - 1;
+            for (int g = 0; g < a.ngroups; ++g) {
+                const unsigned gm = a.grp_mask[g];
+                const unsigned rest = (unsigned)qmask & ~gm;
+                int et = 0;
+                {
+                    int k = 0;
+                    for (int i = 0; i < a.Q; ++i)
+                        if ((rest >> i) & 1u) {
+                            et |= ((t >> k) & 1) << i;
+                            ++k;
+                        }
+                }
+                int pr[A_LOG > 0 ? A_LOG : 1];
+                {
+                    unsigned g2 = gm;
+#pragma unroll
+                    for (int r = 0; r < A_LOG; ++r) {
+                        pr[r] = __builtin_ctz(g2);
+                        g2 &= g2 - 1u;
+                    }
+                }
+                int ej[A];
+#pragma unroll
+                for (int j = 0; j < A; ++j) {
+                    int e = 0;
+#pragma unroll
+                    for (int r = 0; r < A_LOG; ++r)
+                        if ((j >> r) & 1)
+                            e |= 1 << pr[r];
+                    ej[j] = e;
+                }
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    v[j] = lds[lds_pad(et | ej[j])];
+                rx_registers<T, R, A_LOG>(v, a.grp_gates[g], c, s);
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    lds[lds_pad(et | ej[j])] = v[j];
+                __syncthreads();
+            }
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                v[j] = lds[lds_pad(t | (j << a.nthr))];
+        }
+    }
+
+#pragma unroll
+    for (int j = 0; j < A; ++j)
+        p0[roff[j]] = v[j];
+}
+
+// ---- <C> reduction ---------------------------------------------------------
+
+template <typename R, int A_LOG>
+__global__ void __launch_bounds__(256) k_expect(const typename CT<R>::T *__restrict__ sv,
+                                                const TileArgs a, double *__restrict__ partials)
+{
+    typedef typename CT<R>::T T;
+    constexpr int A = 1 << A_LOG;
+    __shared__ double s_c[FQ_MAXQ + 2];
+    __shared__ double s_red[64];
+    __shared__ double s_pc[256];
+    __shared__ double s_p[256];
+    const int t = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const uint64_t tile = blockIdx.x;
+
+    uint64_t base = 0;
+    for (int i = 0; i < a.nh; ++i)
+        base |= ((tile >> i) & 1ull) << a.hpos[i];
+    uint64_t toff = 0;
+    for (int i = 0; i < a.nthr; ++i)
+        toff |= (uint64_t)((t >> i) & 1) << a.tpos[i];
+    const T *p0 = sv + (base + toff);
+    T v[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                o |= 1ull << a.tpos[a.nthr + r];
+        v[j] = p0[o];
+    }
+    cut_tile_prologue(a, tile, s_c, s_red, t, nthreads);
+    __syncthreads();
+    double cr[A_LOG > 0 ? A_LOG : 1];
+    const double cbase = cut_thread_setup<A_LOG>(a, s_c, s_red, t, nthreads, cr);
+    double acc_pc = 0.0, acc_p = 0.0;
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        const double cut = cut_of<A_LOG>(a, cbase, cr, j);
+        const double re = (double)v[j].x, im = (double)v[j].y;
+        const double p = re * re + im * im;
+        acc_pc += p * cut;
+        acc_p += p;
+    }
+    s_pc[t] = acc_pc;
+    s_p[t] = acc_p;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && t + w < nthreads) {
+            s_pc[t] += s_pc[t + w];
+            s_p[t] += s_p[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        partials[2 * tile] = s_pc[0];
+        partials[2 * tile + 1] = s_p[0];
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_finalize(const double *__restrict__ partials,
+                                                   uint64_t n, double *__restrict__ out)
+{
+    __shared__ double s_a[1024];
+    __shared__ double s_b[1024];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (uint64_t i = t; i < n; i += 1024) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
+    }
+    s_a[t] = a;
+    s_b[t] = b;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) {
+            s_a[t] += s_a[t + w];
+            s_b[t] += s_b[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = s_a[0];
+        out[1] = s_b[0];
+    }
+}
+
+// ---- block transposition ---------------------------------------------------
+
+__device__ __forceinline__ uint64_t deposit_rest(uint64_t r, const SwapArgs &a)
+{
+    uint64_t o = 0;
+    for (int i = 0; i < a.nr; ++i)
+        o |= ((r >> i) & 1ull) << a.rpos[i];
+    return o;
+}
+
+// out == 0: the low block is contiguous.  Work-group = one pair of 2^t x 2^t
+// sub-tiles (X <= Y) of the 2^k x 2^k transpose for one rest index.
+template <typename T>
+__global__ void __launch_bounds__(256) k_swap_tiled(T *__restrict__ sv, const SwapArgs a)
+{
+    extern __shared__ __align__(16) unsigned char fq_smem[];
+    const int side = 1 << a.t;
+    const int stride = side + 1;
+    T *la = reinterpret_cast<T *>(fq_smem);
+    T *lb = la + side * stride;
+    const uint64_t lin = blockIdx.x;
+    const uint64_t p = lin % a.npairs;
+    const uint64_t r = lin / a.npairs;
+    // p -> (X <= Y): p = Y(Y+1)/2 + X
+    uint64_t Y = (uint64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while (Y * (Y + 1) / 2 > p)
+        --Y;
+    while ((Y + 1) * (Y + 2) / 2 <= p)
+        ++Y;
+    const uint64_t X = p - Y * (Y + 1) / 2;
+    const uint64_t base = deposit_rest(r, a);
+    const int n = side * side;
+    const int s = a.in;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int xr = i >> a.t, yc = i & (side - 1);
+        la[xr * stride + yc] = sv[base + ((X * side + xr) << s) + (Y * side + yc)];
+        if (X != Y)
+            lb[xr * stride + yc] = sv[base + ((Y * side + xr) << s) + (X * side + yc)];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int xr = i >> a.t, yc = i & (side - 1);
+        if (X != Y) {
+            sv[base + ((X * side + xr) << s) + (Y * side + yc)] = lb[yc * stride + xr];
+            sv[base + ((Y * side + xr) << s) + (X * side + yc)] = la[yc * stride + xr];
+        } else {
+            sv[base + ((X * side + xr) << s) + (Y * side + yc)] = la[yc * stride + xr];
+        }
+    }
+}
+
+// Any out: one thread per (rest, x, y); the x < y threads swap their pair.
+template <typename T>
+__global__ void __launch_bounds__(256) k_swap_generic(T *__restrict__ sv, const SwapArgs a,
+                                                      uint64_t total)
+{
+    const uint64_t lin = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= total)
+        return;
+    const uint64_t km = (1ull << a.k) - 1;
+    const uint64_t y = lin & km;
+    const uint64_t x = (lin >> a.k) & km;
+    if (x >= y)
+        return;
+    const uint64_t r = lin >> (2 * a.k);
+    const uint64_t base = deposit_rest(r, a);
+    const uint64_t i0 = base + (x << a.in) + (y << a.out);
+    const uint64_t i1 = base + (y << a.in) + (x << a.out);
+    const T tmp = sv[i0];
+    sv[i0] = sv[i1];
+    sv[i1] = tmp;
+}
+
+// ---- top-k -----------------------------------------------------------------
+
+__device__ __forceinline__ bool better(double pa, uint64_t ia, double pb, uint64_t ib)
+{
+    return pa > pb || (pa == pb && ia < ib);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_topk(const T *__restrict__ sv, uint64_t n,
+                                              double *__restrict__ cand_p,
+                                              uint64_t *__restrict__ cand_i)
+{
+    constexpr int K = FQ_TOPK_MAX;
+    __shared__ double sp[256 * K];
+    __shared__ uint64_t si[256 * K];
+    double lp[K];
+    uint64_t li[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        lp[k] = -1.0;
+        li[k] = ~0ull;
+    }
+    const int t = threadIdx.x;
+    for (uint64_t b = (uint64_t)blockIdx.x * 256 + t; b < n; b += (uint64_t)gridDim.x * 256) {
+        const T v = sv[b];
+        const double re = (double)v.x, im = (double)v.y;
+        const double p = re * re + im * im;
+        if (better(p, b, lp[K - 1], li[K - 1])) {
+            lp[K - 1] = p;
+            li[K - 1] = b;
+#pragma unroll
+            for (int k = K - 1; k > 0; --k) {
+                if (better(lp[k], li[k], lp[k - 1], li[k - 1])) {
+                    const double tp = lp[k];
+                    lp[k] = lp[k - 1];
+                    lp[k - 1] = tp;
+                    const uint64_t ti = li[k];
+                    li[k] = li[k - 1];
+                    li[k - 1] = ti;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        sp[t * K + k] = lp[k];
+        si[t * K + k] = li[k];
+    }
+    __syncthreads();
+    constexpr int M = 256 * K;
+    for (int kk = 2; kk <= M; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = t; i < M; i += 256) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const bool asc = (i & kk) == 0;
+                    const bool sw = asc ? better(sp[ixj], si[ixj], sp[i], si[i])
+                                        : better(sp[i], si[i], sp[ixj], si[ixj]);
+                    if (sw) {
+                        const double tp = sp[i];
+                        sp[i] = sp[ixj];
+                        sp[ixj] = tp;
+                        const uint64_t ti = si[i];
+                        si[i] = si[ixj];
+                        si[ixj] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t < K) {
+        cand_p[(uint64_t)blockIdx.x * K + t] = sp[t];
+        cand_i[(uint64_t)blockIdx.x * K + t] = si[t];
+    }
+}
+
+// ---- small helpers -------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather(const T *__restrict__ sv,
+                                                const uint64_t *__restrict__ idx, uint64_t n,
+                                                double *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const T v = sv[idx[i]];
+        out[2 * i] = (double)v.x;
+        out[2 * i + 1] = (double)v.y;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero_one(T *__restrict__ sv, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        T v;
+        v.x = (i == 0) ? 1 : 0;
+        v.y = 0;
+        sv[i] = v;
+    }
+}
+
+// ---- launchers ---------------------------------------------------------------
+
+int tile_max_q(int dtype) { return dtype == 0 ? 13 : 14; }
+
+size_t tile_lds_bytes(int dtype, int Q)
+{
+    const size_t elem = dtype == 0 ? 16 : 8;
+    const size_t n = ((size_t)1 << Q) + ((size_t)1 << Q) / 16;
+    return n * elem;
+}
+
+template <typename R, int A_LOG, bool COST, bool FIRST, bool GATES>
+static hipError_t launch_tile_t(void *sv, const TileArgs &a, hipStream_t stream)
+{
+    typedef typename CT<R>::T T;
+    const unsigned threads = 1u << a.nthr;
+    const uint64_t tiles = 1ull << a.nh;
+    const size_t lds = GATES && a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
+    if (lds > 65536) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute(
+                reinterpret_cast<const void *>(&k_tile<R, A_LOG, COST, FIRST, GATES>),
+                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+            if (e != hipSuccess)
+                return e;
+            attr_set = true;
+        }
+    }
+    hipLaunchKernelGGL((k_tile<R, A_LOG, COST, FIRST, GATES>), dim3((unsigned)tiles), dim3(threads),
+                       lds, stream, reinterpret_cast<T *>(sv), a);
+    return hipGetLastError();
+}
+
+template <typename R, int A_LOG>
+static hipError_t launch_tile_flags(bool cost, bool first, bool gates, void *sv, const TileArgs &a,
+                                    hipStream_t stream)
+{
+    if (!cost)
+        return launch_tile_t<R, A_LOG, false, false, true>(sv, a, stream);
+    if (first)
+        return gates ? launch_tile_t<R, A_LOG, true, true, true>(sv, a, stream)
+                     : launch_tile_t<R, A_LOG, true, true, false>(sv, a, stream);
+    return gates ? launch_tile_t<R, A_LOG, true, false, true>(sv, a, stream)
+                 : launch_tile_t<R, A_LOG, true, false, false>(sv, a, stream);
+}
+
+template <typename R>
+static hipError_t launch_tile_r(int a_log, bool cost, bool first, bool gates, void *sv,
+                                const TileArgs &a, hipStream_t stream)
+{
+    switch (a_log) {
+        case 1: return launch_tile_flags<R, 1>(cost, first, gates, sv, a, stream);
+        case 2: return launch_tile_flags<R, 2>(cost, first, gates, sv, a, stream);
+        case 3: return launch_tile_flags<R, 3>(cost, first, gates, sv, a, stream);
+        case 4: return launch_tile_flags<R, 4>(cost, first, gates, sv, a, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, void *sv,
+                       const TileArgs &a, hipStream_t stream)
+{
+    if (!cost && !gates)
+        return hipSuccess;
+    if (dtype == 0)
+        return launch_tile_r<double>(a_log, cost, first, gates, sv, a, stream);
+    return launch_tile_r<float>(a_log, cost, first, gates, sv, a, stream);
+}
+
+template <typename T>
+static hipError_t launch_swap_t(void *sv, const SwapArgs &a, hipStream_t stream)
+{
+    if (a.t >= 0) {
+        const uint64_t nrest = 1ull << a.nr;
+        const uint64_t blocks = a.npairs * nrest;
+        const int side = 1 << a.t;
+        const size_t lds = 2 * (size_t)side * (side + 1) * sizeof(T);
+        hipLaunchKernelGGL(k_swap_tiled<T>, dim3((unsigned)blocks), dim3(256), lds, stream,
+                           reinterpret_cast<T *>(sv), a);
+    } else {
+        const uint64_t total = 1ull << (a.nr + 2 * a.k);
+        const uint64_t blocks = (total + 255) / 256;
+        hipLaunchKernelGGL(k_swap_generic<T>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           reinterpret_cast<T *>(sv), a, total);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_swap(int dtype, void *sv, const SwapArgs &a, hipStream_t stream)
+{
+    return dtype == 0 ? launch_swap_t<double2>(sv, a, stream) : launch_swap_t<float2>(sv, a, stream);
+}
+
+template <typename R, int A_LOG>
+static hipError_t launch_expect_t(const void *sv, const TileArgs &a, double *partials, double *out2,
+                                  hipStream_t stream)
+{
+    typedef typename CT<R>::T T;
+    const uint64_t tiles = 1ull << a.nh;
+    hipLaunchKernelGGL((k_expect<R, A_LOG>), dim3((unsigned)tiles), dim3(1u << a.nthr), 0, stream,
+                       reinterpret_cast<const T *>(sv), a, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, partials, tiles, out2);
+    return hipGetLastError();
+}
+
+hipError_t launch_expect(int dtype, int a_log, const void *sv, const TileArgs &a, double *partials,
+                         double *out2, hipStream_t stream)
+{
+#define FQ_EXP(R)                                                              \
+    switch (a_log) {                                                           \
+        case 1: return launch_expect_t<R, 1>(sv, a, partials, out2, stream);   \
+        case 2: return launch_expect_t<R, 2>(sv, a, partials, out2, stream);   \
+        case 3: return launch_expect_t<R, 3>(sv, a, partials, out2, stream);   \
+        case 4: return launch_expect_t<R, 4>(sv, a, partials, out2, stream);   \
+        default: return hipErrorInvalidValue;                                  \
+    }
+    if (dtype == 0) {
+        FQ_EXP(double)
+    } else {
+        FQ_EXP(float)
+    }
+#undef FQ_EXP
+}
+
+hipError_t launch_topk(int dtype, const void *sv, int N, int nblocks, double *cand_p,
+                       uint64_t *cand_i, hipStream_t stream)
+{
+    const uint64_t n = 1ull << N;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_topk<double2>, dim3(nblocks), dim3(256), 0, stream,
+                           reinterpret_cast<const double2 *>(sv), n, cand_p, cand_i);
+    else
+        hipLaunchKernelGGL(k_topk<float2>, dim3(nblocks), dim3(256), 0, stream,
+                           reinterpret_cast<const float2 *>(sv), n, cand_p, cand_i);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(int dtype, const void *sv, const uint64_t *idx, size_t n, double *out,
+                         hipStream_t stream)
+{
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (n == 0)
+        return hipSuccess;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_gather<double2>, dim3(blocks), dim3(256), 0, stream,
+                           reinterpret_cast<const double2 *>(sv), idx, (uint64_t)n, out);
+    else
+        hipLaunchKernelGGL(k_gather<float2>, dim3(blocks), dim3(256), 0, stream,
+                           reinterpret_cast<const float2 *>(sv), idx, (uint64_t)n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero_one(int dtype, void *sv, int N, hipStream_t stream)
+{
+    const uint64_t n = 1ull << N;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536)
+        blocks = 65536;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_zero_one<double2>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           reinterpret_cast<double2 *>(sv), n);
+    else
+        hipLaunchKernelGGL(k_zero_one<float2>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                           reinterpret_cast<float2 *>(sv), n);
+    return hipGetLastError();
+}
+
+}  // namespace fq

@@ -1,0 +1,166 @@
+/*
+ * forqaoa.h -- C ABI of the MI355X (gfx950) QAOA Max-Cut state-vector library.
+ *
+ * The reference (FOR-QAOA) has no FFI: its hot path is a set of C++ free
+ * functions called by src/Weighted.cpp.  Each entry point below names the
+ * reference function it replaces (file:line, relative to the FOR-QAOA root).
+ * Plain pointers and sizes only; every call returns an int status
+ * (FQ_OK = 0) and fq_last_error() gives the message of the calling thread's
+ * last failure.  The reference exits the process on any error
+ * (include/helper_cuda.hpp:588-596); the `weighted` driver maps a non-zero
+ * status to exit(1) to keep that CLI behaviour.
+ *
+ * Conventions (as the reference):
+ *  - amplitude index b = sum_i z_i 2^i: physical qubit i is bit i;
+ *  - amplitudes are interleaved {re, im}: complex128 = 2 doubles,
+ *    complex64 = 2 floats;
+ *  - the cost phase multiplies psi(b) by exp(i*gamma*s(b)/2) with
+ *    s(b) = sum_{i<j} W_ij * (z_i != z_j ? +1 : -1)  (src/rzz.cu:131-155);
+ *  - the mixer is RX(beta) = exp(-i*beta/2*X) on each target
+ *    (src/rx.cu:46-49, 121).
+ *
+ * Threading: all work of a state is queued on that state's HIP stream; calls
+ * return without waiting except where noted (reductions, copies, fq_sync).
+ * One host thread per state handle; distinct handles may be used from
+ * distinct threads.  A graph may be shared by states used from one thread.
+ */
+#ifndef FORQAOA_H
+#define FORQAOA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FQ_OK 0
+#define FQ_ERR_ARG 1         /* invalid argument */
+#define FQ_ERR_OOM 2         /* device or host allocation failed */
+#define FQ_ERR_BUDGET 3      /* request exceeds the configured memory budget */
+#define FQ_ERR_DEVICE 4      /* HIP runtime / kernel error */
+#define FQ_ERR_UNSUPPORTED 5 /* valid in the reference but not on one GPU (D > 0) */
+
+/* amplitude types */
+#define FQ_C128 0 /* complex128, as the reference (include/state.h:23-31) */
+#define FQ_C64 1  /* complex64 amplitudes; cut values, phases, reductions in FP64 */
+
+/* layer schedules for fq_run */
+#define FQ_SCHED_REFERENCE 0 /* exactly src/Weighted.cpp:115-137: cost, R x RX chunk, R-1 x transposition */
+#define FQ_SCHED_FUSED 1     /* cost fused into the first tile pass; high qubits by strided tiles, no transposition */
+
+/* kernel classes for timing */
+#define FQ_K_COST 0
+#define FQ_K_RX 1
+#define FQ_K_SWAP 2
+#define FQ_K_FUSED 3
+#define FQ_K_REDUCE 4
+#define FQ_K_TOPK 5
+#define FQ_K_ADJOINT 6
+#define FQ_K_COUNT 7
+
+typedef struct fq_state fq_state;
+typedef struct fq_graph fq_graph;
+
+/* Message of this thread's last failed call ("" if none). */
+const char *fq_last_error(void);
+/* ABI version (major*100 + minor). */
+int fq_version(void);
+
+/* ---- state ------------------------------------------------------------ */
+
+/* createMPIState / createState + prepare (src/state.cpp:31-86, 89-159,
+ * 162-196).  N qubits, D log2 devices (must be 0: one GPU), B log2 amplitudes
+ * of the inter-device transfer buffer (validated B <= N-D as README.md:31
+ * states; no buffer is needed on one GPU), C chunk qubits of the RX tile
+ * (C <= 13 for FQ_C128, C <= 14 for FQ_C64: the 2^C tile lives in LDS).
+ * The amplitudes are left uninitialised: the first cost layer writes them. */
+int fq_state_create(int N, int D, int B, int C, int dtype, fq_state **out);
+/* destroyState (include/state.h:69; never called by the reference). */
+int fq_state_destroy(fq_state *st);
+/* Geometry query: any pointer may be NULL. */
+int fq_state_info(const fq_state *st, int *N, int *C, int *dtype, size_t *bytes);
+/* Raw device pointer of the amplitudes (for zero-copy interop). */
+int fq_state_device_ptr(const fq_state *st, void **dptr);
+/* Run the state's work on an external HIP stream (NULL: its own stream). */
+int fq_state_set_stream(fq_state *st, void *hip_stream);
+/* stateInitDeviceState (src/state.cpp:198-219): |0...0>.  Also resets the
+ * qubit map to the identity. */
+int fq_state_init_zero(fq_state *st);
+/* initH (src/rzz.cu:26-28): the launch-control amplitude (1/sqrt 2)^N used
+ * by the first cost layer.  Optional: fq_state_create computes it too. */
+int fq_init_h(fq_state *st);
+
+/* ---- graph ------------------------------------------------------------ */
+
+/* init2DGraph (src/graph.cpp:46-52): dense N x N upper-triangular weights. */
+int fq_graph_create(int N, fq_graph **out);
+/* addEdgeTo2DGraph (src/graph.cpp:54-64): W[min][max] = w; a repeated edge
+ * overwrites; w = 0 means no edge (skipped as src/rzz.cu:134). */
+int fq_graph_add_edge(fq_graph *g, int u, int v, double w);
+/* free2DGraph (src/graph.cpp:66-68). */
+int fq_graph_destroy(fq_graph *g);
+/* Copy the current N*N weight matrix (current labels) to W. */
+int fq_graph_weights(const fq_graph *g, double *W);
+/* mapping_after_swap_conti (src/swap.cu:27-50): relabel after a block swap. */
+int fq_graph_relabel(fq_graph *g, int swap_out, int swap_in, int num_swap);
+
+/* ---- gate operations (one layer's building blocks) --------------------- */
+
+/* rotationCompressionWeighted (src/rzz.cu:173-206): psi(b) *= exp(i*gamma*
+ * s(b)/2) with W in the state's physical labels; is_first != 0 is the
+ * launch-control layer: psi(b) = (1/sqrt2)^N exp(i*gamma*s(b)/2), no read. */
+int fq_cost_layer(fq_state *st, fq_graph *g, double gamma, int is_first);
+/* multiRotateX (src/rx.cu:118-174): RX(beta) on each listed physical qubit.
+ * Targets below C run in one 2^C-amplitude LDS tile pass as the reference;
+ * any other distinct targets < N are also accepted (strided tiles). */
+int fq_multi_rx(fq_state *st, const int *targets, int gate_size, double beta);
+/* swap_gate_conti, local branch (src/swap.cu:104-105, 166-191): exchange
+ * qubit blocks [swap_out, swap_out+k) and [swap_in, swap_in+k) in HBM;
+ * requires swap_out + k <= swap_in and swap_in + k <= N.  Updates the
+ * state's physical->logical qubit map. */
+int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_swap);
+
+/* The driver loop (src/Weighted.cpp:97-137) for P layers with per-layer
+ * angles gammas[p], betas[p].  FQ_SCHED_REFERENCE also relabels g after each
+ * transposition exactly as the driver does (g ends in the state's physical
+ * labels); FQ_SCHED_FUSED leaves the layout and g unchanged. */
+int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
+           int schedule);
+
+/* ---- observables (no reference counterpart: the driver prints timings only,
+ *      src/Weighted.cpp:153-163) ---------------------------------------- */
+
+/* <C> = sum_b |psi_b|^2 Cut(b), W in physical labels; FP64 accumulation,
+ * deterministic order.  norm2 (may be NULL) receives sum |psi|^2.  Waits. */
+int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *norm2);
+/* The k <= 16 most probable basis states, best first (ties: smaller physical
+ * index); idx receives LOGICAL indices (via the qubit map).  Waits. */
+int fq_topk(fq_state *st, int k, uint64_t *idx, double *prob);
+/* Physical -> logical qubit map: map[p] = logical qubit at physical bit p. */
+int fq_qubit_map(const fq_state *st, int *map);
+/* Copy the amplitudes to host memory (bytes = 2^N * element size);
+ * logical != 0 reorders into logical qubit order.  Waits. */
+int fq_state_copy_out(fq_state *st, void *host, size_t bytes, int logical);
+/* Upload amplitudes (physical order) from host memory.  Waits. */
+int fq_state_copy_in(fq_state *st, const void *host, size_t bytes);
+/* Read n amplitudes as complex128 pairs out[2i], out[2i+1]; idx are logical
+ * indices if logical != 0, else physical.  Waits. */
+int fq_state_gather(fq_state *st, const uint64_t *idx, size_t n, double *out, int logical);
+/* Wait for all queued work of the state. */
+int fq_sync(fq_state *st);
+
+/* ---- timing (src/rx.cu:139-157 accumulates rx_gate_time_ms the same way) */
+
+/* Enable/disable per-launch HIP event timing on the state's stream. */
+int fq_timing_enable(fq_state *st, int on);
+/* Accumulated device time (ms), launch count and algorithmic HBM bytes of a
+ * kernel class since the last reset.  Waits for the recorded launches. */
+int fq_timing_get(fq_state *st, int kind, double *ms, int64_t *count, double *bytes);
+int fq_timing_reset(fq_state *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point
+declared in include/forqaoa.h, validates arguments before touching the device,
+and its host-side graph relabel matches the oracle's restatement of
+mapping_after_swap_conti (src/swap.cu:27-50)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from forqaoa import Graph, _lib
+from forqaoa import graphs as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "forqaoa.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fq_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "fq_state_create" in syms and "fq_run" in syms and len(syms) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    missing = [s for s in declared_symbols() if getattr(L, s, None) is None]
+    assert not missing, missing
+
+
+def test_every_declared_symbol_has_a_binding():
+    missing = [s for s in declared_symbols() if s not in _lib.SIGNATURES]
+    assert not missing, missing
+
+
+def _create(N, D, B, C, dtype):
+    h = ctypes.c_void_p()
+    rc = _lib.lib().fq_state_create(N, D, B, C, dtype, ctypes.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize(
+    "args,code",
+    [
+        ((0, 0, 0, 1, 0), _lib.FQ_ERR_ARG),       # N < 1
+        ((37, 0, 10, 10, 0), _lib.FQ_ERR_ARG),    # N too large
+        ((20, 1, 10, 10, 0), _lib.FQ_ERR_UNSUPPORTED),  # D > 0 needs 2^D GPUs
+        ((20, 0, 21, 10, 0), _lib.FQ_ERR_ARG),    # B > N - D (README.md:31)
+        ((20, 0, 10, 14, 0), _lib.FQ_ERR_ARG),    # 2^14 complex128 tile > LDS
+        ((20, 0, 10, 15, 1), _lib.FQ_ERR_ARG),    # 2^15 complex64 tile > LDS
+        ((20, 0, 10, 0, 0), _lib.FQ_ERR_ARG),     # C < 1
+        ((20, 0, 10, 10, 7), _lib.FQ_ERR_ARG),    # unknown dtype
+    ],
+)
+def test_state_create_validates_before_allocating(args, code):
+    rc, h = _create(*args)
+    assert rc == code
+    assert h.value is None
+    assert _lib.lib().fq_last_error().decode()
+
+
+def test_null_handles_are_rejected():
+    L = _lib.lib()
+    assert L.fq_cost_layer(None, None, 0.1, 0) == _lib.FQ_ERR_ARG
+    assert L.fq_swap_conti(None, 0, 1, 1) == _lib.FQ_ERR_ARG
+    assert L.fq_state_destroy(None) == _lib.FQ_OK
+    assert L.fq_graph_destroy(None) == _lib.FQ_OK
+
+
+def test_graph_add_edge_semantics():
+    g = Graph(5)
+    g.add_edge(3, 1, 0.5)
+    g.add_edge(1, 3, 0.25)  # repeated edge overwrites (src/graph.cpp:63)
+    g.add_edge(0, 4, 2.0)
+    W = g.weights()
+    assert W[1, 3] == 0.25 and W[3, 1] == 0 and W[0, 4] == 2.0
+    with pytest.raises(_lib.FQError):
+        g.add_edge(0, 5, 1.0)
+    with pytest.raises(_lib.FQError):
+        g.add_edge(2, 2, 1.0)
+    with pytest.raises(_lib.FQError):
+        g.add_edge(0, 1, float("nan"))
+
+
+@pytest.mark.parametrize("N,out,inn,k", [(8, 0, 3, 3), (12, 0, 5, 5), (9, 1, 5, 4), (30, 0, 13, 13), (30, 0, 26, 4)])
+def test_graph_relabel_matches_oracle(N, out, inn, k):
+    edges = G.gnp(N, 0.6, 17 + N, "uniform")
+    g = Graph(N, edges)
+    W = O.dense_w(N, edges)
+    g.relabel(out, inn, k)
+    O.relabel(W, out, inn, k)
+    np.testing.assert_array_equal(g.weights(), W)
+
+
+def test_graph_relabel_rejects_overlap():
+    g = Graph(8)
+    with pytest.raises(_lib.FQError) as e:
+        g.relabel(0, 2, 3)
+    assert e.value.code == _lib.FQ_ERR_ARG
+
+
+def test_driver_rejects_multi_device_like_reference_exit():
+    exe = os.path.join(ROOT, "for-qaoa_amd", "host", "weighted")
+    assert os.path.exists(exe), "driver not built"
+    r = subprocess.run([exe, "1", "20", "1", "10", "10"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "D=1" in r.stderr
+    r = subprocess.run([exe, "1", "20", "0", "10", "10", "--dtype", "c32"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2

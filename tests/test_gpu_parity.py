@@ -1,0 +1,259 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on
+the same seeded inputs, and against closed forms at full size.
+
+Tolerances: complex128 results within 1e-12 relative
+of the oracle (states: ||psi_gpu - psi_oracle||_2 / ||psi_oracle||_2; <C>:
+relative); complex64 results within 1e-5 of the complex128 oracle.  Integer
+work (qubit maps, index permutations) is exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import golden_util as GU
+import pyoracle as O
+import refmath as R
+from forqaoa import Graph, State
+from forqaoa import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c128": 1e-12, "c64": 1e-5}
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(a.astype(np.complex128) - b) / np.linalg.norm(b))
+
+
+def random_state(N, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=1 << N) + 1j * rng.normal(size=1 << N)
+    return v / np.linalg.norm(v)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("N", [3, 12, 16])
+def test_cost_layer_vs_oracle(dtype, N):
+    edges = G.gnp(N, 0.5, 40 + N, "uniform")
+    W = O.dense_w(N, edges)
+    st = State(N, C=min(N, 10), dtype=dtype)
+    g = Graph(N, edges)
+    st.cost_layer(g, 0.83, True)
+    ref = np.zeros(1 << N, dtype=np.complex128)
+    O.cost_layer(ref, W, 0.83, True)
+    assert rel_err(st.copy_out(), ref) < TOL[dtype]
+    st.cost_layer(g, -2.1, False)
+    O.cost_layer(ref, W, -2.1, False)
+    assert rel_err(st.copy_out(), ref) < TOL[dtype]
+
+
+def test_cost_layer_integer_weights_large_angle():
+    N = 14
+    edges = G.gnp(N, 0.6, 5, "int16")
+    W = O.dense_w(N, edges)
+    st = State(N, C=10)
+    g = Graph(N, edges)
+    st.cost_layer(g, 2.9, True)
+    ref = np.zeros(1 << N, dtype=np.complex128)
+    O.cost_layer(ref, W, 2.9, True)
+    assert rel_err(st.copy_out(), ref) < 1e-13
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("targets", [list(range(10)), [0, 1, 2, 3], [9], [2, 5, 7], [0]])
+def test_multi_rx_chunk_vs_oracle(dtype, targets):
+    N, C = 15, 10
+    psi = random_state(N, 3)
+    st = State(N, C=C, dtype=dtype)
+    st.copy_in(psi)
+    st.multi_rx(targets, 0.71)
+    ref = psi.copy()
+    O.multi_rx(ref, N, C, targets, 0.71)
+    assert rel_err(st.copy_out(), ref) < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("targets", [[12, 13, 14], list(range(3, 16)), [0, 15], list(range(16))])
+def test_multi_rx_any_targets_vs_oracle(dtype, targets):
+    N = 16
+    psi = random_state(N, 4)
+    st = State(N, C=10, dtype=dtype)
+    st.copy_in(psi)
+    st.multi_rx(targets, -1.3)
+    ref = psi.copy()
+    O.multi_rx(ref, N, N, targets, -1.3)  # oracle chunk = whole state: any target
+    assert rel_err(st.copy_out(), ref) < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("N,out,inn,k", [(16, 0, 8, 8), (16, 0, 10, 5), (17, 0, 13, 4), (15, 0, 6, 3),
+                                         (14, 0, 7, 2), (14, 0, 5, 1), (16, 2, 9, 6), (12, 1, 4, 3)])
+def test_swap_vs_oracle_bitexact(dtype, N, out, inn, k):
+    psi = random_state(N, 5)
+    st = State(N, C=10, dtype=dtype)
+    st.copy_in(psi)
+    before = st.copy_out()
+    st.swap_conti(out, inn, k)
+    ref128 = before.astype(np.complex128)
+    O.swap_conti(ref128, N, out, inn, k)
+    got = st.copy_out()
+    np.testing.assert_array_equal(got, ref128.astype(got.dtype))
+    qmap = list(range(N))
+    for t in range(k):
+        qmap[out + t], qmap[inn + t] = qmap[inn + t], qmap[out + t]
+    assert list(st.qubit_map()) == qmap
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_run_reference_schedule_n20_vs_oracle(dtype):
+    d = GU.instances()["n20_gnp05"]
+    fx = GU.oracle_n20()
+    N, C = d["N"], d["C"]
+    sv, qmap, W = O.run_weighted(N, C, d["edges"], d["gammas"], d["betas"])
+    st = State(N, C=C, dtype=dtype)
+    g = Graph(N, d["edges"])
+    st.run(g, d["gammas"], d["betas"], "reference")
+    assert list(st.qubit_map()) == list(qmap) == fx["qmap"]
+    np.testing.assert_array_equal(g.weights(), W)  # graph relabelled as the driver does
+    got = st.copy_out()
+    assert rel_err(got, sv) < TOL[dtype]
+    s = np.array(fx["sample_idx"])
+    ref_s = np.array(fx["sample_re"]) + 1j * np.array(fx["sample_im"])
+    assert np.max(np.abs(got[s] - ref_s)) < TOL[dtype] * 2.0 ** (-N / 2) * 10
+    ex, n2 = st.expect_cut(g)
+    assert ex == pytest.approx(fx["expect_cut"], rel=TOL[dtype])
+    assert n2 == pytest.approx(1.0, abs=TOL[dtype] * 10)
+    idx, prob = st.topk(8)
+    np.testing.assert_allclose(prob, fx["top8_prob"], rtol=TOL[dtype] * 100)
+    # the basis states with those probabilities (ties: complement pairs)
+    lg = O.to_logical(sv, qmap)
+    p_ref = np.abs(lg) ** 2
+    np.testing.assert_allclose(p_ref[idx.astype(np.int64)], prob, rtol=TOL[dtype] * 100)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("name", ["n20_gnp05", "s10_gnp", "s9_gnp_c4", "s5_gnp_c5"])
+def test_run_fused_schedule_vs_oracle(dtype, name):
+    d = GU.instances()[name]
+    N, C = d["N"], d["C"]
+    sv, qmap, W = O.run_weighted(N, C, d["edges"], d["gammas"], d["betas"])
+    st = State(N, C=C, dtype=dtype)
+    g = Graph(N, d["edges"])
+    st.run(g, d["gammas"], d["betas"], "fused")
+    assert list(st.qubit_map()) == list(range(N))
+    got = st.copy_out(logical=True)
+    assert rel_err(got, O.to_logical(sv, qmap)) < TOL[dtype]
+    ex, _ = st.expect_cut(g)
+    assert ex == pytest.approx(O.expect_cut(sv, W), rel=TOL[dtype])
+
+
+@pytest.mark.parametrize("name", ["s4_gnp", "s6_reg3", "s7_gnp_int", "s8_gnp", "s10_reg4_c10"])
+@pytest.mark.parametrize("schedule", ["reference", "fused"])
+def test_run_vs_dense_fixtures(name, schedule):
+    d = GU.instances()[name]
+    psi = GU.dense_small()[name]
+    st = State(d["N"], C=d["C"])
+    g = Graph(d["N"], d["edges"])
+    st.run(g, d["gammas"], d["betas"], schedule)
+    assert rel_err(st.copy_out(logical=True), psi) < 1e-12
+
+
+@pytest.mark.parametrize("N,C", [(1, 1), (2, 1), (2, 2), (3, 2), (5, 3), (6, 13)])
+@pytest.mark.parametrize("schedule", ["reference", "fused"])
+def test_tiny_states(N, C, schedule):
+    edges = G.complete(N, 0.7) if N > 1 else []
+    g_, b_ = G.angles(3, N)
+    sv, qmap, W = O.run_weighted(N, C, edges, g_, b_)
+    st = State(N, C=C)
+    g = Graph(N, edges)
+    st.run(g, g_, b_, schedule)
+    assert rel_err(st.copy_out(logical=True), O.to_logical(sv, qmap)) < 1e-12
+
+
+def test_layer_by_layer_calls_match_run():
+    # the driver loop written against the per-op entry points
+    d = GU.instances()["s10_gnp"]
+    N, C = d["N"], d["C"]
+    st = State(N, C=C)
+    g = Graph(N, d["edges"])
+    sizes = [min(C, N - i) for i in range(0, N, C)]
+    for p in range(len(d["gammas"])):
+        st.rotationCompressionWeighted(d["gammas"][p], g, p == 0)
+        s_in = 0
+        for i, sz in enumerate(sizes):
+            st.multiRotateX(list(range(C)), sz, d["betas"][p])
+            if i < len(sizes) - 1:
+                s_in += sz
+                st.swap_gate_conti(0, s_in, sizes[i + 1])
+                g.mapping_after_swap_conti(0, s_in, sizes[i + 1])
+    sv, qmap, _ = O.run_weighted(N, C, d["edges"], d["gammas"], d["betas"])
+    assert list(st.qubit_map()) == list(qmap)
+    assert rel_err(st.copy_out(), sv) < 1e-12
+
+
+def test_init_zero_and_norm():
+    st = State(12, C=8)
+    st.init_zero()
+    v = st.copy_out()
+    assert v[0] == 1 and np.count_nonzero(v) == 1
+
+
+# ---- full size (N=30): closed forms instead of the oracle -------------------
+
+def _kn(N, label_prefix):
+    for c in GU.kn_cases():
+        if c["N"] == N and c["label"].startswith(label_prefix):
+            return c
+    raise KeyError(label_prefix)
+
+
+@pytest.mark.parametrize("schedule,dtype", [("fused", "c128"), ("reference", "c128"), ("fused", "c64")])
+@pytest.mark.parametrize("label", ["validate_state.py recipe", "random P=5"])
+def test_n30_complete_graph_closed_form(schedule, dtype, label):
+    c = _kn(30, label)
+    N = 30
+    st = State(N, C=13 if dtype == "c128" else 14, dtype=dtype)
+    g = Graph(N, G.complete(N))
+    st.run(g, c["gammas"], c["betas"], schedule)
+    rng = np.random.default_rng(11)
+    idx = rng.integers(0, 1 << N, size=8192, dtype=np.uint64)
+    got = st.gather(idx, logical=True)
+    want = c["phi"][GU.popcount(idx)]
+    scale = 2.0 ** (-N / 2)
+    assert np.max(np.abs(got - want)) / scale < TOL[dtype] * 10
+    ex, n2 = st.expect_cut(g)
+    assert ex == pytest.approx(c["expect_cut"], rel=TOL[dtype])
+    assert n2 == pytest.approx(1.0, abs=TOL[dtype] * 10)
+
+
+@pytest.mark.parametrize("name", ["n30_reg3", "n30_gnp05"])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_n30_single_layer_analytic_expectation(name, dtype):
+    d = GU.instances()[name]
+    p1 = GU.p1_analytic()[name]
+    st = State(30, C=13 if dtype == "c128" else 14, dtype=dtype)
+    g = Graph(30, d["edges"])
+    st.run(g, [p1["gamma"]], [p1["beta"]], "fused")
+    ex, n2 = st.expect_cut(g)
+    assert ex == pytest.approx(p1["expect_cut"], rel=TOL[dtype])
+
+
+def test_n30_reference_and_fused_schedules_agree():
+    d = GU.instances()["n30_gnp05"]
+    N = 30
+    a = State(N, C=13)
+    ga = Graph(N, d["edges"])
+    a.run(ga, d["gammas"], d["betas"], "fused")
+    ea, _ = a.expect_cut(ga)
+    rng = np.random.default_rng(2)
+    idx = rng.integers(0, 1 << N, size=4096, dtype=np.uint64)
+    va = a.gather(idx, logical=True)
+    a.destroy()
+    b = State(N, C=13)
+    gb = Graph(N, d["edges"])
+    b.run(gb, d["gammas"], d["betas"], "reference")
+    eb, _ = b.expect_cut(gb)
+    vb = b.gather(idx, logical=True)
+    assert ea == pytest.approx(eb, rel=1e-12)
+    assert np.max(np.abs(va - vb)) < 1e-12 * 2.0 ** (-N / 2) * 10
