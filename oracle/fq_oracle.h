@@ -76,6 +76,15 @@ void oq_topk(int N, const oq_cplx *sv, int k, uint64_t *idx, double *prob);
  * bit qmap[p]. */
 void oq_to_logical(int N, const oq_cplx *sv, const int *qmap, oq_cplx *out);
 
+/* Timing sample of one layer p >= 1 of the driver loop at full size N (cost
+ * read-modify-write, then RX / swap / relabel per subcircuit as
+ * src/Weighted.cpp:119-137), restricted to the first 1/2^f of each
+ * operation's independent work units: amplitudes of the cost phase, 2^C
+ * chunks of each RX pass, rest indices of each swap.  Every unit costs the
+ * same, so a full layer takes 2^f times as long.  sv must span 2^N
+ * amplitudes (only the touched pages are materialised).  W is relabelled. */
+int oq_layer_sample(int N, int C, double *W, double gamma, double beta, oq_cplx *sv, int f);
+
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int oq_num_threads(void);
 

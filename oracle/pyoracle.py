@@ -47,6 +47,9 @@ def lib():
         L.oq_topk.argtypes = [ctypes.c_int, vp, ctypes.c_int, up, dp]
         L.oq_to_logical.argtypes = [ctypes.c_int, vp, ip, vp]
         L.oq_num_threads.restype = ctypes.c_int
+        L.oq_layer_sample.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, vp,
+                                      ctypes.c_int]
+        L.oq_layer_sample.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -155,3 +158,12 @@ def topk(sv, k):
 
 def num_threads():
     return lib().oq_num_threads()
+
+
+def layer_sample(N, C, W, gamma, beta, sv, f):
+    """One reference-schedule layer at size N on the first 1/2^f of each
+    operation's work units (oq_layer_sample)."""
+    assert W.flags.c_contiguous and W.dtype == np.float64
+    rc = lib().oq_layer_sample(N, C, _dp(W), gamma, beta, sv.ctypes.data, f)
+    if rc:
+        raise ValueError("oq_layer_sample failed")
