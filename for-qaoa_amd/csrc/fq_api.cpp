@@ -71,6 +71,8 @@ struct fq_state {
     double *d_cand_p = nullptr;
     uint64_t *d_cand_i = nullptr;
     int cand_blocks = 0;
+    double2 *d_phase = nullptr;  // per-launch cost phase factors
+    size_t phase_cap = 0;        // entries
     bool timing = false;
     std::vector<TimingRec> pending;
     std::vector<hipEvent_t> pool;
@@ -166,6 +168,31 @@ static Geom make_geom(int N, const std::vector<int> &tile, int a_log)
     return g;
 }
 
+// Group consecutive index bits that land on consecutive positions; returns
+// the number of runs (may exceed FQ_MAX_RUNS: then r is not usable).
+static int make_runs(const uint8_t *pos, int n, BitRuns &r)
+{
+    int cnt = 0;
+    r.n = 0;
+    for (int i = 0; i < n;) {
+        int j = i + 1;
+        while (j < n && pos[j] == pos[j - 1] + 1)
+            ++j;
+        if (cnt < FQ_MAX_RUNS)
+            r.run[cnt] = (uint32_t)i | ((uint32_t)(j - i) << 8) | ((uint32_t)pos[i] << 16);
+        cnt++;
+        i = j;
+    }
+    r.n = cnt < FQ_MAX_RUNS ? cnt : FQ_MAX_RUNS;
+    return cnt;
+}
+
+static int geom_runs(const Geom &g)
+{
+    BitRuns r;
+    return std::max(make_runs(g.hpos, g.nh, r), make_runs(g.tpos, g.nthr, r));
+}
+
 static void geom_to_args(const Geom &g, TileArgs &a)
 {
     memset(&a, 0, sizeof(a));
@@ -173,8 +200,10 @@ static void geom_to_args(const Geom &g, TileArgs &a)
     a.Q = g.Q;
     a.nthr = g.nthr;
     a.nh = g.nh;
-    memcpy(a.tpos, g.tpos, sizeof(a.tpos));
-    memcpy(a.hpos, g.hpos, sizeof(a.hpos));
+    make_runs(g.hpos, g.nh, a.hruns);
+    make_runs(g.tpos, g.nthr, a.truns);
+    for (int r = 0; r < g.a_log; ++r)
+        a.rpos[r] = g.tpos[g.nthr + r];
 }
 
 static std::string geom_key(const Geom &g)
@@ -476,6 +505,7 @@ extern "C" int fq_state_destroy(fq_state *st)
     hipFree(st->d_out2);
     hipFree(st->d_cand_p);
     hipFree(st->d_cand_i);
+    hipFree(st->d_phase);
     if (st->own)
         hipStreamDestroy(st->own);
     delete st;
@@ -629,21 +659,33 @@ static std::vector<int> low_tile(int Q)
     return t;
 }
 
-// Cost-phase launch arguments: angle, launch-control amplitude and the
-// register-register phase factors exp(i*gamma*rr[j]).
+// Cost-phase launch arguments: angle and launch-control amplitude.
 static void set_cost(TileArgs &a, double gamma, double init_amp)
 {
     a.gamma = gamma;
     a.init_amp = init_amp;
     a.has_rr = 0;
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < 16; ++j)
         if (a.plan.rr[j] != 0)
             a.has_rr = 1;
-        double sn, cs;
-        sincos(gamma * a.plan.rr[j], &sn, &cs);
-        a.rr_c[j] = cs;
-        a.rr_s[j] = sn;
+}
+
+// Phase factors for a cost-phase tile launch (grows the state's table; a
+// larger table needs the previous launches that read the old one to finish).
+static int prepare_phases(fq_state *st, TileArgs &a, int a_log)
+{
+    const size_t need = phase_table_entries(a, a_log);
+    if (need > st->phase_cap) {
+        FQ_HIP(hipStreamSynchronize(st->stream));
+        hipFree(st->d_phase);
+        st->d_phase = nullptr;
+        st->phase_cap = 0;
+        if (hipMalloc(&st->d_phase, need * sizeof(double2)) != hipSuccess)
+            return set_err(FQ_ERR_OOM, "phase table allocation of %zu bytes failed", need * sizeof(double2));
+        st->phase_cap = need;
     }
+    FQ_HIP(launch_phase_table(a, a_log, st->d_phase, st->stream));
+    return FQ_OK;
 }
 
 // Cost phase alone: 256 threads x 16 amplitudes per work-group, no LDS.
@@ -657,9 +699,11 @@ static int cost_pass(fq_state *st, fq_graph *g, double gamma, bool first)
     if (int rc = get_plan(g, geom, &a.plan))
         return rc;
     set_cost(a, gamma, st->init_amp);
+    if (int rc = prepare_phases(st, a, a_log))
+        return rc;
     const double bytes = (first ? 1.0 : 2.0) * (double)st->bytes;
     return timed(st, FQ_K_COST, bytes, [&] {
-        return launch_tile(st->dtype, a_log, true, first, false, st->sv, a, st->stream);
+        return launch_tile(st->dtype, a_log, true, first, false, st->sv, a, st->d_phase, st->stream);
     });
 }
 
@@ -670,6 +714,8 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     const int Q = (int)tile.size();
     const int a_log = tile_a_log(st->dtype, Q);
     Geom geom = make_geom(st->N, tile, a_log);
+    if (geom_runs(geom) > FQ_MAX_RUNS)
+        return set_err(FQ_ERR_ARG, "internal: tile geometry needs more than %d bit runs", FQ_MAX_RUNS);
     TileArgs a;
     geom_to_args(geom, a);
     uint32_t gm = 0;
@@ -692,10 +738,13 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
         if (int rc = get_plan(g, geom, &a.plan))
             return rc;
         set_cost(a, gamma, st->init_amp);
+        if (int rc = prepare_phases(st, a, a_log))
+            return rc;
     }
     const double bytes = (first ? 1.0 : 2.0) * (double)st->bytes;
     return timed(st, kind, bytes, [&] {
-        return launch_tile(st->dtype, a_log, cost, first, true, st->sv, a, st->stream);
+        return launch_tile(st->dtype, a_log, cost, first, true, st->sv, a, cost ? st->d_phase : nullptr,
+                           st->stream);
     });
 }
 
@@ -718,11 +767,20 @@ static int rx_targets(fq_state *st, std::vector<int> targets, double beta)
         std::vector<int> chunk;
         while (pos < targets.size() && (int)chunk.size() < per)
             chunk.push_back(targets[pos++]);
-        std::vector<int> tile = chunk;
-        for (int p = 0; (int)tile.size() < Qmax && p < N; ++p)
-            if (std::find(chunk.begin(), chunk.end(), p) == chunk.end())
-                tile.push_back(p);
-        std::sort(tile.begin(), tile.end());
+        std::vector<int> tile;
+        for (;;) {
+            // the chunk's targets plus the lowest other qubits as carriers
+            tile = chunk;
+            for (int p = 0; (int)tile.size() < Qmax && p < N; ++p)
+                if (std::find(chunk.begin(), chunk.end(), p) == chunk.end())
+                    tile.push_back(p);
+            std::sort(tile.begin(), tile.end());
+            const int a_log = tile_a_log(st->dtype, (int)tile.size());
+            if (chunk.size() == 1 || geom_runs(make_geom(N, tile, a_log)) <= FQ_MAX_RUNS)
+                break;
+            chunk.pop_back();  // scattered targets: fewer per pass
+            --pos;
+        }
         if (int rc = tile_pass(st, tile, chunk, beta, nullptr, 0, false, false, FQ_K_RX))
             return rc;
     }
@@ -806,6 +864,11 @@ struct FusedPass {
     std::vector<int> tile, gates;
 };
 
+// Tile passes of one fused layer.  Pass 1 owns the lowest Qmax qubits
+// (contiguous tile, cost phase fused).  The remaining qubits are spread evenly
+// over the fewest strided passes; every strided tile also carries the lowest
+// qubits (at least 128 contiguous bytes per run) so that its HBM accesses
+// stay coalesced, and a pass with fewer new qubits carries more.
 static std::vector<FusedPass> fused_passes(int N, int dtype)
 {
     std::vector<FusedPass> ps;
@@ -814,10 +877,15 @@ static std::vector<FusedPass> fused_passes(int N, int dtype)
     p1.tile = low_tile(Qmax);
     p1.gates = p1.tile;
     ps.push_back(p1);
-    int next = Qmax;
+    const int rest = N - Qmax;
+    if (rest <= 0)
+        return ps;
     const int cmin = std::min(carriers_for(dtype), Qmax - 1);
-    while (next < N) {
-        const int n_new = std::min(Qmax - cmin, N - next);
+    const int per_max = Qmax - cmin;
+    const int npass = (rest + per_max - 1) / per_max;
+    int next = Qmax;
+    for (int k = 0; k < npass; ++k) {
+        const int n_new = rest / npass + (k < rest % npass ? 1 : 0);
         const int c = std::min(Qmax - n_new, next);
         FusedPass p;
         for (int i = 0; i < c; ++i)

@@ -9,7 +9,8 @@
 
 #define FQ_MAXN 40         // qubits (2^40 complex64 = 8 TiB: the HBM limit bites first)
 #define FQ_MAXQ 14         // tile qubits held in LDS (2^14 complex64 = 128 KiB)
-#define FQ_MAX_GROUPS 16   // LDS re-mapping phases per tile pass
+#define FQ_MAX_GROUPS 5    // LDS re-mapping phases per tile pass
+#define FQ_MAX_RUNS 8      // bit runs of a scattered index
 #define FQ_TOPK_MAX 16
 
 namespace fq {
@@ -42,12 +43,20 @@ struct PlanDev {
     double rr[16];         // register-register cut contributions
 };
 
+// Index bits [src, src+len) go to physical positions [dst, dst+len); one run
+// packed per word as src | len << 8 | dst << 16.
+struct BitRuns {
+    int n;
+    uint32_t run[FQ_MAX_RUNS];
+};
+
 // One tile pass: load 2^Q amplitudes per work-group (tile bits tpos[], high
 // bits hpos[]), optionally apply the cost phase, RX on the gate bits, store.
 struct TileArgs {
     int N, Q, nthr, nh;
-    uint8_t tpos[16];
-    uint8_t hpos[FQ_MAXN];
+    BitRuns hruns;           // tile index -> high positions
+    BitRuns truns;           // thread index -> positions of the thread bits
+    uint8_t rpos[4];         // positions of the register bits
     uint32_t gate_mask;      // over tile bits
     int ngroups;
     uint16_t grp_mask[FQ_MAX_GROUPS];  // tile-bit mask of each LDS phase (A_LOG bits)
@@ -55,7 +64,6 @@ struct TileArgs {
     double rx_c, rx_s;       // cos(beta/2), sin(beta/2) (host-computed, src/rx.cu:121)
     double gamma, init_amp;  // cost phase angle, launch-control amplitude
     int has_rr;              // any register-register edge in the plan
-    double rr_c[16], rr_s[16];  // cos/sin(gamma * plan.rr[j]), host-evaluated
     PlanDev plan;
 };
 
@@ -70,7 +78,10 @@ struct SwapArgs {
 // Launchers (fq_kernels.hip).  All enqueue on `stream` and return the HIP
 // launch status.
 hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, void *sv,
-                       const TileArgs &a, hipStream_t stream);
+                       const TileArgs &a, const double2 *phase_tab, hipStream_t stream);
+// Phase factors of a cost-phase launch into tab (size phase_table_entries).
+hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, hipStream_t stream);
+size_t phase_table_entries(const TileArgs &a, int a_log);
 hipError_t launch_swap(int dtype, void *sv, const SwapArgs &a, hipStream_t stream);
 hipError_t launch_expect(int dtype, int a_log, const void *sv, const TileArgs &a, double *partials,
                          double *out2, hipStream_t stream);

@@ -71,10 +71,20 @@ __device__ __forceinline__ void cmul_by(T &v, R c, R s)
     v.y = x * s + y * c;
 }
 
-// Per-tile part of the cut decomposition: c_i(zH) for every tile bit and the
-// partial sums of K(zH) (summed by the caller after a barrier).
+// Barrier that orders LDS accesses only: outstanding global loads (the next
+// tile's prefetch) stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Per-tile part of the cut decomposition, computed by at most two waves:
+// c_i(zH) for every tile bit (lanes i < Q) and K(zH) (one wave, butterfly
+// reduction, lane 0 writes).  Published by the caller's barrier.
 __device__ __forceinline__ void cut_tile_prologue(const TileArgs &a, uint64_t tile, double *s_c,
-                                                  double *s_red, int t, int nthreads)
+                                                  double *s_K, int t, int nthreads)
 {
     const PlanDev &p = a.plan;
     for (int i = t; i < a.Q; i += nthreads) {
@@ -83,48 +93,83 @@ __device__ __forceinline__ void cut_tile_prologue(const TileArgs &a, uint64_t ti
             acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
         s_c[i] = p.cH[i] - 2.0 * acc;
     }
-    const int lanes = nthreads < 64 ? nthreads : 64;
-    if (t < lanes) {
+    if (nthreads >= 64) {
+        const int kw = nthreads >= 128 ? 64 : 0;
+        if (t >= kw && t < kw + 64) {
+            const int lane = t - kw;
+            double acc = 0.0;
+            for (int e = lane; e < p.n_hh; e += 64)
+                acc += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
+            for (int h = lane; h < a.nh; h += 64)
+                acc += p.omega[h] * (double)((tile >> h) & 1ull);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1)
+                acc += __shfl_xor(acc, off, 64);
+            if (lane == 0)
+                *s_K = acc;
+        }
+    } else if (t == 0) {
         double acc = 0.0;
-        for (int e = t; e < p.n_hh; e += lanes)
+        for (int e = 0; e < p.n_hh; ++e)
             acc += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
-        for (int h = t; h < a.nh; h += lanes)
+        for (int h = 0; h < a.nh; ++h)
             acc += p.omega[h] * (double)((tile >> h) & 1ull);
-        s_red[t] = acc;
+        *s_K = acc;
     }
 }
 
-// Per-thread part: base cut of the thread's amplitudes and the coefficient of
-// each register bit.  Call after the barrier that publishes s_c / s_red.
-template <int A_LOG>
-__device__ __forceinline__ double cut_thread_setup(const TileArgs &a, const double *s_c,
-                                                   const double *s_red, int t, int nthreads,
-                                                   double *cr)
+// Phase table of one cost-phase launch (the tile kernel does no transcendental
+// work).  Layout: for each tile, Q+1 factors
+//   tab[tile*(Q+1) + i] = exp(i*gamma*c_i(zH))          i < Q
+//   tab[tile*(Q+1) + Q] = exp(i*gamma*(K(zH) - W/2))
+// then, per thread t of the tile kernel and register bit r,
+//   tab[ntiles*(Q+1) + t]                  = exp(i*gamma*thr[t])
+//   tab[ntiles*(Q+1) + ((1+r) << nthr) + t] = exp(i*gamma*d_r(t))
+// and finally the 16 register-register factors exp(i*gamma*rr[j]),
+// so that exp(i*gamma*s/2) = exp(i*gamma*(Cut - W/2)) of every amplitude is a
+// product of table entries.
+__global__ void __launch_bounds__(256) k_phase_table(const TileArgs a, int a_log,
+                                                     double2 *__restrict__ tab)
 {
     const PlanDev &p = a.plan;
-    const int lanes = nthreads < 64 ? nthreads : 64;
-    double K = 0.0;
-    for (int l = 0; l < lanes; ++l)
-        K += s_red[l];
-    double base = K + p.thr[t];
-    for (int i = 0; i < a.nthr; ++i)
-        base += s_c[i] * (double)((t >> i) & 1);
-#pragma unroll
-    for (int r = 0; r < A_LOG; ++r)
-        cr[r] = s_c[a.nthr + r] + p.thr[((1 + r) << a.nthr) + t];
-    return base;
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t q1 = (uint64_t)a.Q + 1;
+    const uint64_t ntile_e = (1ull << a.nh) * q1;
+    const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
+    double angle;
+    if (gid < ntile_e) {
+        const uint64_t tile = gid / q1;
+        const int i = (int)(gid - tile * q1);
+        if (i < a.Q) {
+            double acc = 0.0;
+            for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
+                acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
+            angle = a.gamma * (p.cH[i] - 2.0 * acc);
+        } else {
+            double K = 0.0;
+            for (int e = 0; e < p.n_hh; ++e)
+                K += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
+            for (int h = 0; h < a.nh; ++h)
+                K += p.omega[h] * (double)((tile >> h) & 1ull);
+            angle = a.gamma * (K - 0.5 * p.wtot);
+        }
+    } else if (gid < ntile_e + nthr_e) {
+        angle = a.gamma * p.thr[gid - ntile_e];
+    } else if (gid < ntile_e + nthr_e + 16) {
+        angle = a.gamma * p.rr[gid - ntile_e - nthr_e];
+    } else {
+        return;
+    }
+    double sn, cs;
+    sincos(angle, &sn, &cs);
+    tab[gid] = make_double2(cs, sn);
 }
 
-// Cut of register j (j is a compile-time constant after unrolling).
-template <int A_LOG>
-__device__ __forceinline__ double cut_of(const TileArgs &a, double base, const double *cr, int j)
+__device__ __forceinline__ void zmul(double &x, double &y, double2 f)
 {
-    double cut = base + a.plan.rr[j];
-#pragma unroll
-    for (int r = 0; r < A_LOG; ++r)
-        if ((j >> r) & 1)
-            cut += cr[r];
-    return cut;
+    const double nx = x * f.x - y * f.y;
+    y = x * f.y + y * f.x;
+    x = nx;
 }
 
 // exp(i*theta) with theta = s*gamma/2 as src/rzz.cu:146 (FP64 sincos).
@@ -143,25 +188,63 @@ __device__ __forceinline__ void phase_of(double theta, float &c, float &s)
     sincosf((float)r, &s, &c);
 }
 
-template <typename R, int A_LOG, bool COST, bool FIRST, bool GATES>
-__global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a)
+// Scatter the bits of a packed index to physical positions, as runs:
+// index bits [src, src+len) go to positions [dst, dst+len).  Unrolled with a
+// fixed bound so the run descriptors stay in scalar registers.
+__device__ __forceinline__ uint64_t unpack_runs(uint64_t x, const BitRuns &r)
+{
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
+        if (k < r.n) {
+            const uint32_t d = r.run[k];
+            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
+        }
+    }
+    return o;
+}
+
+__device__ __forceinline__ uint64_t tile_base(const TileArgs &a, uint64_t tile)
+{
+    return unpack_runs(tile, a.hruns);
+}
+
+// deposit the low bits of x into the set bits of mask (q-bit masks)
+template <int QT>
+__device__ __forceinline__ int deposit(int x, unsigned mask, int q)
+{
+    int e = 0, k = 0;
+    const int qq = QT > 0 ? QT : q;
+#pragma unroll
+    for (int i = 0; i < (QT > 0 ? QT : FQ_MAXQ); ++i) {
+        if (i < qq && ((mask >> i) & 1u)) {
+            e |= ((x >> k) & 1) << i;
+            ++k;
+        }
+    }
+    return e;
+}
+
+// Persistent tile pass.  Each work-group walks tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ...; while a tile goes through its cost phase and
+// LDS phases, the next tile's amplitudes are already loading into registers.
+// QT > 0 fixes the tile width at compile time (the fused passes' full-width
+// tiles); QT == 0 reads it from the arguments.
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES>
+__global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a,
+                                                            const double2 *__restrict__ ptab)
 {
     typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
+    constexpr int MAXG = FQ_MAX_GROUPS;
     extern __shared__ __align__(16) unsigned char fq_smem[];
-    __shared__ double s_c[FQ_MAXQ + 2];
-    __shared__ double s_red[64];
 
+    const int Q = QT > 0 ? QT : a.Q;
+    const int nthr = QT > 0 ? QT - A_LOG : a.nthr;
     const int t = threadIdx.x;
-    const int nthreads = blockDim.x;
-    const uint64_t tile = blockIdx.x;
+    const uint64_t ntiles = 1ull << a.nh;
 
-    uint64_t base = 0;
-    for (int i = 0; i < a.nh; ++i)
-        base |= ((tile >> i) & 1ull) << a.hpos[i];
-    uint64_t toff = 0;
-    for (int i = 0; i < a.nthr; ++i)
-        toff |= (uint64_t)((t >> i) & 1) << a.tpos[i];
+    const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     uint64_t roff[A];
 #pragma unroll
     for (int j = 0; j < A; ++j) {
@@ -169,118 +252,158 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
 #pragma unroll
         for (int r = 0; r < A_LOG; ++r)
             if ((j >> r) & 1)
-                o |= 1ull << a.tpos[a.nthr + r];
+                o |= 1ull << a.rpos[r];
         roff[j] = o;
     }
-    T *const p0 = sv + (base + toff);
+    // per-thread LDS slots of every remapping phase (padded), once per launch;
+    // lds_pad is additive over disjoint bit sets: pad(x | y) = pad(x) + pad(y)
+    int etp[MAXG];
+    const int nat = lds_pad(t);
+    if (GATES) {
+        const unsigned qmask = (1u << Q) - 1u;
+#pragma unroll
+        for (int g = 0; g < MAXG; ++g)
+            etp[g] = g < a.ngroups ? lds_pad(deposit<QT>(t, qmask & ~(unsigned)a.grp_mask[g], Q)) : 0;
+    }
 
+    const uint64_t q1 = (uint64_t)Q + 1;
+    const double2 *thr_tab = COST ? ptab + (ntiles * q1) : nullptr;
+    const double2 *rr_tab = COST ? thr_tab + ((size_t)(1 + A_LOG) << nthr) : nullptr;
+
+    uint64_t tile = blockIdx.x;
     T v[A];
-    if (!FIRST) {
+    if (!FIRST && tile < ntiles) {
+        const T *p0 = sv + (tile_base(a, tile) + toff);
 #pragma unroll
         for (int j = 0; j < A; ++j)
             v[j] = p0[roff[j]];
     }
+    for (; tile < ntiles; tile += gridDim.x) {
+        const uint64_t next = tile + gridDim.x;
 
-    if (COST) {
-        // exp(i*gamma*s/2) with s = 2*Cut - W, applied as a product of factors:
-        // the thread's base angle, one factor per register bit set in j, and
-        // the register-register term (host-evaluated per launch).
-        cut_tile_prologue(a, tile, s_c, s_red, t, nthreads);
-        __syncthreads();
-        double cr[A_LOG > 0 ? A_LOG : 1];
-        const double cbase = cut_thread_setup<A_LOG>(a, s_c, s_red, t, nthreads, cr);
-        const double theta0 = (2.0 * cbase - a.plan.wtot) * a.gamma / 2;
-        R c0, s0;
-        phase_of(theta0, c0, s0);
-        if (FIRST) {
-            const R amp = (R)a.init_amp;
-            c0 *= amp;
-            s0 *= amp;
-        }
+        // Cost phase first (few registers live), then the prefetch.
+        // exp(i*gamma*s/2), s = 2*Cut - W, of register j is the product of the
+        // thread's base factor, one factor per register bit set in j and the
+        // register-register term; all factors come from the phase table.
+        if (COST) {
+            const double2 *f_t = ptab + tile * q1;
+            const double2 f_thr = thr_tab[t];
+            double x = f_thr.x, y = f_thr.y;
+            zmul(x, y, f_t[Q]);
 #pragma unroll
-        for (int j = 0; j < A; ++j) {
+            for (int i = 0; i < (QT > 0 ? QT - A_LOG : FQ_MAXQ); ++i) {
+                if (QT > 0 || i < nthr) {
+                    const double2 f = f_t[i];
+                    const double nx = x * f.x - y * f.y;
+                    const double ny = x * f.y + y * f.x;
+                    const bool on = (t >> i) & 1;
+                    x = on ? nx : x;
+                    y = on ? ny : y;
+                }
+            }
             if (FIRST) {
-                v[j].x = c0;
-                v[j].y = s0;
-            } else {
-                cmul_by(v[j], c0, s0);
+                x *= a.init_amp;
+                y *= a.init_amp;
+            }
+            const R c0 = (R)x, s0 = (R)y;
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                if (FIRST) {
+                    v[j].x = c0;
+                    v[j].y = s0;
+                } else {
+                    cmul_by(v[j], c0, s0);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < A_LOG; ++r) {
+                const double2 fr = thr_tab[((1 + r) << nthr) + t];
+                double fx = fr.x, fy = fr.y;
+                zmul(fx, fy, f_t[nthr + r]);
+                const R cf = (R)fx, sf = (R)fy;
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    if ((j >> r) & 1)
+                        cmul_by(v[j], cf, sf);
+            }
+            if (a.has_rr) {
+#pragma unroll
+                for (int j = 0; j < A; ++j) {
+                    const double2 f = rr_tab[j];
+                    cmul_by(v[j], (R)f.x, (R)f.y);
+                }
             }
         }
-#pragma unroll
-        for (int r = 0; r < A_LOG; ++r) {
-            R cf, sf;
-            phase_of(a.gamma * cr[r], cf, sf);
-#pragma unroll
-            for (int j = 0; j < A; ++j)
-                if ((j >> r) & 1)
-                    cmul_by(v[j], cf, sf);
-        }
-        if (a.has_rr) {
-#pragma unroll
-            for (int j = 0; j < A; ++j)
-                cmul_by(v[j], (R)a.rr_c[j], (R)a.rr_s[j]);
-        }
-    }
 
-    if (GATES) {
-        const R c = (R)a.rx_c, s = (R)a.rx_s;
-        rx_registers<T, R, A_LOG>(v, a.gate_mask >> a.nthr, c, s);
-        if (a.ngroups > 0) {
-            T *lds = reinterpret_cast<T *>(fq_smem);
+        T w[A];
+        if (!FIRST && next < ntiles) {
+            const T *p1 = sv + (tile_base(a, next) + toff);
 #pragma unroll
             for (int j = 0; j < A; ++j)
-                lds[lds_pad(t | (j << a.nthr))] = v[j];
-            __syncthreads();
-            const int qmask = (1 << a.Q) - 1;
-            for (int g = 0; g < a.ngroups; ++g) {
-                const unsigned gm = a.grp_mask[g];
-                const unsigned rest = (unsigned)qmask & ~gm;
-                int et = 0;
-                {
-                    int k = 0;
-                    for (int i = 0; i < a.Q; ++i)
-                        if ((rest >> i) & 1u) {
-                            et |= ((t >> k) & 1) << i;
-                            ++k;
-                        }
-                }
-                int pr[A_LOG > 0 ? A_LOG : 1];
-                {
+                w[j] = p1[roff[j]];
+        }
+
+        if (GATES) {
+            const R c = (R)a.rx_c, s = (R)a.rx_s;
+            rx_registers<T, R, A_LOG>(v, a.gate_mask >> nthr, c, s);
+            if (a.ngroups > 0) {
+                T *lds = reinterpret_cast<T *>(fq_smem);
+                // own slots: the previous tile's final read used the same
+                // slots, so no barrier is needed before this write
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    lds[nat + lds_pad(j << nthr)] = v[j];
+                lds_barrier();
+                // rolled loop: unrolling lets the compiler hoist every
+                // phase's addresses out of the tile loop and spill
+#pragma unroll 1
+                for (int g = 0; g < a.ngroups; ++g) {
+                    const int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
+                    const unsigned gm = a.grp_mask[g];
+                    int pr[A_LOG > 0 ? A_LOG : 1];
                     unsigned g2 = gm;
 #pragma unroll
                     for (int r = 0; r < A_LOG; ++r) {
                         pr[r] = __builtin_ctz(g2);
                         g2 &= g2 - 1u;
                     }
-                }
-                int ej[A];
+                    int ej[A];
 #pragma unroll
-                for (int j = 0; j < A; ++j) {
-                    int e = 0;
+                    for (int j = 0; j < A; ++j) {
+                        int e = 0;
 #pragma unroll
-                    for (int r = 0; r < A_LOG; ++r)
-                        if ((j >> r) & 1)
-                            e |= 1 << pr[r];
-                    ej[j] = e;
+                        for (int r = 0; r < A_LOG; ++r)
+                            if ((j >> r) & 1)
+                                e |= 1 << pr[r];
+                        ej[j] = lds_pad(e);
+                    }
+#pragma unroll
+                    for (int j = 0; j < A; ++j)
+                        v[j] = lds[eg + ej[j]];
+                    rx_registers<T, R, A_LOG>(v, a.grp_gates[g], c, s);
+#pragma unroll
+                    for (int j = 0; j < A; ++j)
+                        lds[eg + ej[j]] = v[j];
+                    lds_barrier();
                 }
 #pragma unroll
                 for (int j = 0; j < A; ++j)
-                    v[j] = lds[lds_pad(et | ej[j])];
-                rx_registers<T, R, A_LOG>(v, a.grp_gates[g], c, s);
-#pragma unroll
-                for (int j = 0; j < A; ++j)
-                    lds[lds_pad(et | ej[j])] = v[j];
-                __syncthreads();
+                    v[j] = lds[nat + lds_pad(j << nthr)];
             }
+        }
+
+        {
+            T *const p0 = sv + (tile_base(a, tile) + toff);
 #pragma unroll
             for (int j = 0; j < A; ++j)
-                v[j] = lds[lds_pad(t | (j << a.nthr))];
+                p0[roff[j]] = v[j];
+        }
+        if (!FIRST && next < ntiles) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                v[j] = w[j];
         }
     }
-
-#pragma unroll
-    for (int j = 0; j < A; ++j)
-        p0[roff[j]] = v[j];
 }
 
 // ---- <C> reduction ---------------------------------------------------------
@@ -292,20 +415,15 @@ __global__ void __launch_bounds__(256) k_expect(const typename CT<R>::T *__restr
     typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
     __shared__ double s_c[FQ_MAXQ + 2];
-    __shared__ double s_red[64];
+    __shared__ double s_K;
     __shared__ double s_pc[256];
     __shared__ double s_p[256];
     const int t = threadIdx.x;
     const int nthreads = blockDim.x;
     const uint64_t tile = blockIdx.x;
 
-    uint64_t base = 0;
-    for (int i = 0; i < a.nh; ++i)
-        base |= ((tile >> i) & 1ull) << a.hpos[i];
-    uint64_t toff = 0;
-    for (int i = 0; i < a.nthr; ++i)
-        toff |= (uint64_t)((t >> i) & 1) << a.tpos[i];
-    const T *p0 = sv + (base + toff);
+    const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
+    const T *p0 = sv + (tile_base(a, tile) + toff);
     T v[A];
 #pragma unroll
     for (int j = 0; j < A; ++j) {
@@ -313,21 +431,30 @@ __global__ void __launch_bounds__(256) k_expect(const typename CT<R>::T *__restr
 #pragma unroll
         for (int r = 0; r < A_LOG; ++r)
             if ((j >> r) & 1)
-                o |= 1ull << a.tpos[a.nthr + r];
+                o |= 1ull << a.rpos[r];
         v[j] = p0[o];
     }
-    cut_tile_prologue(a, tile, s_c, s_red, t, nthreads);
+    cut_tile_prologue(a, tile, s_c, &s_K, t, nthreads);
     __syncthreads();
+    double cbase = s_K + a.plan.thr[t];
+    for (int i = 0; i < a.nthr; ++i)
+        cbase += s_c[i] * (double)((t >> i) & 1);
     double cr[A_LOG > 0 ? A_LOG : 1];
-    const double cbase = cut_thread_setup<A_LOG>(a, s_c, s_red, t, nthreads, cr);
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r)
+        cr[r] = s_c[a.nthr + r] + a.plan.thr[((1 + r) << a.nthr) + t];
     double acc_pc = 0.0, acc_p = 0.0;
 #pragma unroll
     for (int j = 0; j < A; ++j) {
-        const double cut = cut_of<A_LOG>(a, cbase, cr, j);
+        double cut = cbase + a.plan.rr[j];
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                cut += cr[r];
         const double re = (double)v[j].x, im = (double)v[j].y;
-        const double p = re * re + im * im;
-        acc_pc += p * cut;
-        acc_p += p;
+        const double pr = re * re + im * im;
+        acc_pc += pr * cut;
+        acc_p += pr;
     }
     s_pc[t] = acc_pc;
     s_p[t] = acc_p;
@@ -560,63 +687,103 @@ size_t tile_lds_bytes(int dtype, int Q)
     return n * elem;
 }
 
-template <typename R, int A_LOG, bool COST, bool FIRST, bool GATES>
-static hipError_t launch_tile_t(void *sv, const TileArgs &a, hipStream_t stream)
+static int device_cus()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES>
+static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab, hipStream_t stream)
 {
     typedef typename CT<R>::T T;
     const unsigned threads = 1u << a.nthr;
     const uint64_t tiles = 1ull << a.nh;
     const size_t lds = GATES && a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
+    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES>);
     if (lds > 65536) {
         static bool attr_set = false;
         if (!attr_set) {
-            hipError_t e = hipFuncSetAttribute(
-                reinterpret_cast<const void *>(&k_tile<R, A_LOG, COST, FIRST, GATES>),
-                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024 - 1024);
             if (e != hipSuccess)
                 return e;
             attr_set = true;
         }
     }
-    hipLaunchKernelGGL((k_tile<R, A_LOG, COST, FIRST, GATES>), dim3((unsigned)tiles), dim3(threads),
-                       lds, stream, reinterpret_cast<T *>(sv), a);
+    // persistent grid: as many work-groups as can be resident at once
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint64_t resident = (uint64_t)per_cu * device_cus();
+    const uint64_t grid = tiles < resident ? tiles : resident;
+    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES>), dim3((unsigned)grid), dim3(threads),
+                       lds, stream, reinterpret_cast<T *>(sv), a, ptab);
     return hipGetLastError();
 }
 
-template <typename R, int A_LOG>
+template <typename R, int A_LOG, int QT>
 static hipError_t launch_tile_flags(bool cost, bool first, bool gates, void *sv, const TileArgs &a,
-                                    hipStream_t stream)
+                                    const double2 *ptab, hipStream_t stream)
 {
     if (!cost)
-        return launch_tile_t<R, A_LOG, false, false, true>(sv, a, stream);
+        return launch_tile_t<R, A_LOG, QT, false, false, true>(sv, a, ptab, stream);
     if (first)
-        return gates ? launch_tile_t<R, A_LOG, true, true, true>(sv, a, stream)
-                     : launch_tile_t<R, A_LOG, true, true, false>(sv, a, stream);
-    return gates ? launch_tile_t<R, A_LOG, true, false, true>(sv, a, stream)
-                 : launch_tile_t<R, A_LOG, true, false, false>(sv, a, stream);
+        return gates ? launch_tile_t<R, A_LOG, QT, true, true, true>(sv, a, ptab, stream)
+                     : launch_tile_t<R, A_LOG, QT, true, true, false>(sv, a, ptab, stream);
+    return gates ? launch_tile_t<R, A_LOG, QT, true, false, true>(sv, a, ptab, stream)
+                 : launch_tile_t<R, A_LOG, QT, true, false, false>(sv, a, ptab, stream);
 }
 
 template <typename R>
 static hipError_t launch_tile_r(int a_log, bool cost, bool first, bool gates, void *sv,
-                                const TileArgs &a, hipStream_t stream)
+                                const TileArgs &a, const double2 *ptab, hipStream_t stream)
 {
+    // compile-time tile widths for the full-width tiles of the fused and
+    // cost passes; anything else takes the generic instance
+    if (sizeof(R) == 8 && a_log == 3 && a.Q == 13 && gates)
+        return launch_tile_flags<R, 3, 13>(cost, first, gates, sv, a, ptab, stream);
+    if (sizeof(R) == 4 && a_log == 4 && a.Q == 14 && gates)
+        return launch_tile_flags<R, 4, 14>(cost, first, gates, sv, a, ptab, stream);
+    if (a_log == 4 && a.Q == 12 && !gates)
+        return launch_tile_flags<R, 4, 12>(cost, first, gates, sv, a, ptab, stream);
     switch (a_log) {
-        case 1: return launch_tile_flags<R, 1>(cost, first, gates, sv, a, stream);
-        case 2: return launch_tile_flags<R, 2>(cost, first, gates, sv, a, stream);
-        case 3: return launch_tile_flags<R, 3>(cost, first, gates, sv, a, stream);
-        case 4: return launch_tile_flags<R, 4>(cost, first, gates, sv, a, stream);
+        case 1: return launch_tile_flags<R, 1, 0>(cost, first, gates, sv, a, ptab, stream);
+        case 2: return launch_tile_flags<R, 2, 0>(cost, first, gates, sv, a, ptab, stream);
+        case 3: return launch_tile_flags<R, 3, 0>(cost, first, gates, sv, a, ptab, stream);
+        case 4: return launch_tile_flags<R, 4, 0>(cost, first, gates, sv, a, ptab, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, void *sv,
-                       const TileArgs &a, hipStream_t stream)
+                       const TileArgs &a, const double2 *ptab, hipStream_t stream)
 {
     if (!cost && !gates)
         return hipSuccess;
     if (dtype == 0)
-        return launch_tile_r<double>(a_log, cost, first, gates, sv, a, stream);
-    return launch_tile_r<float>(a_log, cost, first, gates, sv, a, stream);
+        return launch_tile_r<double>(a_log, cost, first, gates, sv, a, ptab, stream);
+    return launch_tile_r<float>(a_log, cost, first, gates, sv, a, ptab, stream);
+}
+
+size_t phase_table_entries(const TileArgs &a, int a_log)
+{
+    return ((size_t)1 << a.nh) * (size_t)(a.Q + 1) + ((size_t)(1 + a_log) << a.nthr) + 16;
+}
+
+hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, hipStream_t stream)
+{
+    const size_t n = phase_table_entries(a, a_log);
+    const size_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_phase_table, dim3((unsigned)blocks), dim3(256), 0, stream, a, a_log, tab);
+    return hipGetLastError();
 }
 
 template <typename T>
