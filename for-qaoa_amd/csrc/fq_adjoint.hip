@@ -1,0 +1,500 @@
+// fq_adjoint.hip -- adjoint (reverse-uncompute) gradient of <C> for the fused
+// schedule.  No reference counterpart: the reference computes no expectation
+// and no gradient (src/Weighted.cpp:153-163 prints timings only).
+//
+// Forward: psi_k = M(beta_k) U(gamma_k) psi_{k-1}, psi_0 = |+>^N,
+//   U(g) = exp(i g S/2), S = diag(2 Cut - W)   (src/rzz.cu:146)
+//   M(b) = exp(-i b/2 B), B = sum_j X_j          (src/rx.cu:46-49)
+// E = <psi_P| Cut |psi_P>.  With lambda = Cut psi_P and both states walked
+// back layer by layer (psi <- U^+ M^+ psi, lambda <- U^+ M^+ lambda):
+//   dE/dbeta_k  =  Im <lambda_k | B | psi_k>
+//   dE/dgamma_k = -Im <lambda'_k | S | phi_k>,   lambda'_k = M^+ lambda_k,
+//                                                phi_k = M^+ psi_k.
+// <lambda|X_j|psi> is invariant under any RX applied to both states, so each
+// term is accumulated while qubit j's pairs sit in registers during the
+// uncompute pass; <lambda|S|phi> commutes with the diagonal U and is
+// accumulated in the pass that applies U^+.  FP64 accumulators, fixed-order
+// reductions.
+#include "fq_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace fq {
+
+template <typename R> struct CTA;
+template <> struct CTA<double> { typedef double2 T; };
+template <> struct CTA<float> { typedef float2 T; };
+
+__device__ __forceinline__ int lds_pad_a(int e) { return e + (e >> 4); }
+
+__device__ __forceinline__ void lds_barrier_a()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ uint64_t unpack_runs_a(uint64_t x, const BitRuns &r)
+{
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
+        if (k < r.n) {
+            const uint32_t d = r.run[k];
+            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
+        }
+    }
+    return o;
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ void rx_pair_a(T &q0, T &q1, R c, R s)
+{
+    T o0, o1;
+    o0.x = q0.x * c + q1.y * s;
+    o0.y = q0.y * c - q1.x * s;
+    o1.x = q0.y * s + q1.x * c;
+    o1.y = -q0.x * s + q1.y * c;
+    q0 = o0;
+    q1 = o1;
+}
+
+// Im(conj(l0) p1 + conj(l1) p0): the pair's contribution to Im<lambda|X_j|psi>
+template <typename T>
+__device__ __forceinline__ double im_x_pair(const T &l0, const T &l1, const T &p0, const T &p1)
+{
+    return ((double)l0.x * (double)p1.y - (double)l0.y * (double)p1.x) +
+           ((double)l1.x * (double)p0.y - (double)l1.y * (double)p0.x);
+}
+
+template <typename T, typename R, int A_LOG>
+__device__ __forceinline__ void rx_both(T *vp, T *vl, unsigned mask, R c, R s, double &acc)
+{
+    constexpr int A = 1 << A_LOG;
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r) {
+        if ((mask >> r) & 1u) {
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                if (!((j >> r) & 1)) {
+                    const int j1 = j | (1 << r);
+                    acc += im_x_pair(vl[j], vl[j1], vp[j], vp[j1]);
+                    rx_pair_a<T, R>(vp[j], vp[j1], c, s);
+                    rx_pair_a<T, R>(vl[j], vl[j1], c, s);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ void cmul_a(T &v, R c, R s)
+{
+    const R x = v.x, y = v.y;
+    v.x = x * c - y * s;
+    v.y = x * s + y * c;
+}
+
+// One uncompute pass over a tile geometry for psi and lambda together:
+// RX(rx angle) on the gate bits (accumulating Im<lambda|X_j|psi>), then, if
+// COSTBW, Im<lambda|S|phi> from the per-amplitude cut value and the diagonal
+// phase from the phase table (built for -gamma) on both.  partials[2*block]
+// and [2*block+1] receive the work-group's two sums.
+template <typename R, int A_LOG, bool COSTBW>
+__global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restrict__ psi,
+                                                    typename CTA<R>::T *__restrict__ lam,
+                                                    const TileArgs a,
+                                                    const double2 *__restrict__ ptab,
+                                                    double *__restrict__ partials)
+{
+    typedef typename CTA<R>::T T;
+    constexpr int A = 1 << A_LOG;
+    extern __shared__ __align__(16) unsigned char fq_smem_a[];
+    __shared__ double s_c[FQ_MAXQ + 2];
+    __shared__ double s_K;
+    __shared__ double s_red[2][8];
+
+    const int Q = a.Q, nthr = a.nthr;
+    const int t = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const uint64_t ntiles = 1ull << a.nh;
+    const uint64_t toff = unpack_runs_a((uint64_t)t, a.truns);
+    uint64_t roff[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                o |= 1ull << a.rpos[r];
+        roff[j] = o;
+    }
+    const size_t lds_elems = ((size_t)1 << Q) + ((size_t)1 << Q) / 16;
+    T *ldsP = reinterpret_cast<T *>(fq_smem_a);
+    T *ldsL = ldsP + lds_elems;
+    const R c = (R)a.rx_c, s = (R)a.rx_s;
+    const int qmask = (1 << Q) - 1;
+    const uint64_t q1 = (uint64_t)Q + 1;
+    const double2 *thr_tab = COSTBW ? ptab + ntiles * q1 : nullptr;
+    const double2 *rr_tab = COSTBW ? thr_tab + ((size_t)(1 + A_LOG) << nthr) : nullptr;
+
+    double accX = 0.0, accG = 0.0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t base = unpack_runs_a(tile, a.hruns) + toff;
+        T vp[A], vl[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+            vp[j] = psi[base + roff[j]];
+            vl[j] = lam[base + roff[j]];
+        }
+        if (COSTBW) {
+            // per-tile cut coefficients (published by the first LDS barrier)
+            for (int i = t; i < Q; i += nthreads) {
+                double acc = 0.0;
+                for (int e = a.plan.csr_off[i]; e < a.plan.csr_off[i + 1]; ++e)
+                    acc += a.plan.csr_w[e] * (double)((tile >> a.plan.csr_h[e]) & 1ull);
+                s_c[i] = a.plan.cH[i] - 2.0 * acc;
+            }
+            if (t == 0) {
+                double K = 0.0;
+                for (int e = 0; e < a.plan.n_hh; ++e)
+                    K += a.plan.hh_w[e] *
+                         (double)(((tile >> a.plan.hh_u[e]) ^ (tile >> a.plan.hh_v[e])) & 1ull);
+                for (int h = 0; h < a.nh; ++h)
+                    K += a.plan.omega[h] * (double)((tile >> h) & 1ull);
+                s_K = K;
+            }
+        }
+        rx_both<T, R, A_LOG>(vp, vl, a.gate_mask >> nthr, c, s, accX);
+        const int nat = lds_pad_a(t);
+        if (a.ngroups > 0) {
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                ldsP[nat + lds_pad_a(j << nthr)] = vp[j];
+                ldsL[nat + lds_pad_a(j << nthr)] = vl[j];
+            }
+        }
+        lds_barrier_a();
+        for (int g = 0; g < a.ngroups; ++g) {
+            const unsigned gm = a.grp_mask[g];
+            const unsigned rest = (unsigned)qmask & ~gm;
+            int et = 0;
+            {
+                int k = 0;
+                for (int i = 0; i < Q; ++i)
+                    if ((rest >> i) & 1u) {
+                        et |= ((t >> k) & 1) << i;
+                        ++k;
+                    }
+            }
+            const int etp = lds_pad_a(et);
+            int pr[A_LOG > 0 ? A_LOG : 1];
+            unsigned g2 = gm;
+#pragma unroll
+            for (int r = 0; r < A_LOG; ++r) {
+                pr[r] = __builtin_ctz(g2);
+                g2 &= g2 - 1u;
+            }
+            int ej[A];
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                int e = 0;
+#pragma unroll
+                for (int r = 0; r < A_LOG; ++r)
+                    if ((j >> r) & 1)
+                        e |= 1 << pr[r];
+                ej[j] = lds_pad_a(e);
+            }
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                vp[j] = ldsP[etp + ej[j]];
+                vl[j] = ldsL[etp + ej[j]];
+            }
+            rx_both<T, R, A_LOG>(vp, vl, a.grp_gates[g], c, s, accX);
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                ldsP[etp + ej[j]] = vp[j];
+                ldsL[etp + ej[j]] = vl[j];
+            }
+            lds_barrier_a();
+        }
+        if (a.ngroups > 0) {
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                vp[j] = ldsP[nat + lds_pad_a(j << nthr)];
+                vl[j] = ldsL[nat + lds_pad_a(j << nthr)];
+            }
+        }
+        if (COSTBW) {
+            // cut value of each register: base + register coefficients + rr
+            double cbase = s_K + a.plan.thr[t];
+            for (int i = 0; i < nthr; ++i)
+                cbase += s_c[i] * (double)((t >> i) & 1);
+            double cr[A_LOG > 0 ? A_LOG : 1];
+#pragma unroll
+            for (int r = 0; r < A_LOG; ++r)
+                cr[r] = s_c[nthr + r] + a.plan.thr[((1 + r) << nthr) + t];
+            // phase factors exp(-i gamma s/2) from the table
+            const double2 *f_t = ptab + tile * q1;
+            const double2 f_thr = thr_tab[t];
+            double x = f_thr.x, y = f_thr.y;
+            {
+                const double2 f = f_t[Q];
+                const double nx = x * f.x - y * f.y;
+                y = x * f.y + y * f.x;
+                x = nx;
+            }
+            for (int i = 0; i < nthr; ++i) {
+                const double2 f = f_t[i];
+                const double nx = x * f.x - y * f.y;
+                const double ny = x * f.y + y * f.x;
+                const bool on = (t >> i) & 1;
+                x = on ? nx : x;
+                y = on ? ny : y;
+            }
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                double cut = cbase + a.plan.rr[j];
+#pragma unroll
+                for (int r = 0; r < A_LOG; ++r)
+                    if ((j >> r) & 1)
+                        cut += cr[r];
+                const double sj = 2.0 * cut - a.plan.wtot;
+                // Im(conj(l) * s * p)
+                accG += sj * ((double)vl[j].x * (double)vp[j].y - (double)vl[j].y * (double)vp[j].x);
+                cmul_a(vp[j], (R)x, (R)y);
+                cmul_a(vl[j], (R)x, (R)y);
+            }
+#pragma unroll
+            for (int r = 0; r < A_LOG; ++r) {
+                const double2 fr = thr_tab[((1 + r) << nthr) + t];
+                const double2 fq = f_t[nthr + r];
+                const double fx = fr.x * fq.x - fr.y * fq.y;
+                const double fy = fr.x * fq.y + fr.y * fq.x;
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    if ((j >> r) & 1) {
+                        cmul_a(vp[j], (R)fx, (R)fy);
+                        cmul_a(vl[j], (R)fx, (R)fy);
+                    }
+            }
+            if (a.has_rr) {
+#pragma unroll
+                for (int j = 0; j < A; ++j) {
+                    const double2 f = rr_tab[j];
+                    cmul_a(vp[j], (R)f.x, (R)f.y);
+                    cmul_a(vl[j], (R)f.x, (R)f.y);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+            psi[base + roff[j]] = vp[j];
+            lam[base + roff[j]] = vl[j];
+        }
+        // the next tile's first LDS writes reuse this thread's own slots; the
+        // cut coefficients are rewritten only after every thread passed the
+        // barrier below
+        lds_barrier_a();
+    }
+    // fixed-order work-group reduction: waves by shuffle, then 16 slots
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        accX += __shfl_xor(accX, off, 64);
+        accG += __shfl_xor(accG, off, 64);
+    }
+    const int wave = t >> 6, lane = t & 63;
+    if (lane == 0) {
+        s_red[0][wave] = accX;
+        s_red[1][wave] = accG;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double x = 0.0, g = 0.0;
+        for (int w = 0; w < (nthreads + 63) / 64; ++w) {
+            x += s_red[0][w];
+            g += s_red[1][w];
+        }
+        partials[2 * blockIdx.x] = x;
+        partials[2 * blockIdx.x + 1] = g;
+    }
+}
+
+// lambda = Cut * psi, and the partial sums of sum |psi|^2 Cut and sum |psi|^2.
+template <typename R, int A_LOG>
+__global__ void __launch_bounds__(256) k_cut_mul(const typename CTA<R>::T *__restrict__ psi,
+                                                 typename CTA<R>::T *__restrict__ lam, const TileArgs a,
+                                                 double *__restrict__ partials)
+{
+    typedef typename CTA<R>::T T;
+    constexpr int A = 1 << A_LOG;
+    __shared__ double s_c[FQ_MAXQ + 2];
+    __shared__ double s_K;
+    __shared__ double s_pc[256];
+    __shared__ double s_p[256];
+    const int t = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const uint64_t tile = blockIdx.x;
+    const uint64_t base = unpack_runs_a(tile, a.hruns) + unpack_runs_a((uint64_t)t, a.truns);
+    for (int i = t; i < a.Q; i += nthreads) {
+        double acc = 0.0;
+        for (int e = a.plan.csr_off[i]; e < a.plan.csr_off[i + 1]; ++e)
+            acc += a.plan.csr_w[e] * (double)((tile >> a.plan.csr_h[e]) & 1ull);
+        s_c[i] = a.plan.cH[i] - 2.0 * acc;
+    }
+    if (t == 0) {
+        double K = 0.0;
+        for (int e = 0; e < a.plan.n_hh; ++e)
+            K += a.plan.hh_w[e] * (double)(((tile >> a.plan.hh_u[e]) ^ (tile >> a.plan.hh_v[e])) & 1ull);
+        for (int h = 0; h < a.nh; ++h)
+            K += a.plan.omega[h] * (double)((tile >> h) & 1ull);
+        s_K = K;
+    }
+    __syncthreads();
+    double cbase = s_K + a.plan.thr[t];
+    for (int i = 0; i < a.nthr; ++i)
+        cbase += s_c[i] * (double)((t >> i) & 1);
+    double cr[A_LOG > 0 ? A_LOG : 1];
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r)
+        cr[r] = s_c[a.nthr + r] + a.plan.thr[((1 + r) << a.nthr) + t];
+    double acc_pc = 0.0, acc_p = 0.0;
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                o |= 1ull << a.rpos[r];
+        double cut = cbase + a.plan.rr[j];
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                cut += cr[r];
+        const T v = psi[base + o];
+        const double re = (double)v.x, im = (double)v.y;
+        const double pr = re * re + im * im;
+        acc_pc += pr * cut;
+        acc_p += pr;
+        T w;
+        w.x = (R)(re * cut);
+        w.y = (R)(im * cut);
+        lam[base + o] = w;
+    }
+    s_pc[t] = acc_pc;
+    s_p[t] = acc_p;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && t + w < nthreads) {
+            s_pc[t] += s_pc[t + w];
+            s_p[t] += s_p[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        partials[2 * tile] = s_pc[0];
+        partials[2 * tile + 1] = s_p[0];
+    }
+}
+
+static int cus_a()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+size_t pair_lds_bytes(int dtype, int Q)
+{
+    const size_t elem = dtype == 0 ? 16 : 8;
+    return 2 * (((size_t)1 << Q) + ((size_t)1 << Q) / 16) * elem;
+}
+
+template <typename R, int A_LOG, bool COSTBW>
+static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const double2 *ptab,
+                                double *partials, unsigned grid, hipStream_t stream)
+{
+    typedef typename CTA<R>::T T;
+    const size_t lds = pair_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q);
+    const void *fn = reinterpret_cast<const void *>(&k_pair_pass<R, A_LOG, COSTBW>);
+    if (lds > 65536) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024 - 1024);
+            if (e != hipSuccess)
+                return e;
+            attr_set = true;
+        }
+    }
+    hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW>), dim3(grid), dim3(1u << a.nthr), lds, stream,
+                       reinterpret_cast<T *>(psi), reinterpret_cast<T *>(lam), a, ptab, partials);
+    return hipGetLastError();
+}
+
+unsigned pair_grid(const TileArgs &a)
+{
+    const uint64_t tiles = 1ull << a.nh;
+    const uint64_t cus = (uint64_t)cus_a();
+    return (unsigned)(tiles < cus ? tiles : cus);
+}
+
+hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
+                       const double2 *ptab, double *partials, hipStream_t stream)
+{
+    const unsigned grid = pair_grid(a);
+#define FQ_PAIR(R, AL)                                                                             \
+    return costbw ? launch_pair_t<R, AL, true>(psi, lam, a, ptab, partials, grid, stream)         \
+                  : launch_pair_t<R, AL, false>(psi, lam, a, ptab, partials, grid, stream);
+    if (dtype == 0) {
+        switch (a_log) {
+            case 1: FQ_PAIR(double, 1)
+            case 2: FQ_PAIR(double, 2)
+            case 3: FQ_PAIR(double, 3)
+            default: return hipErrorInvalidValue;
+        }
+    }
+    switch (a_log) {
+        case 1: FQ_PAIR(float, 1)
+        case 2: FQ_PAIR(float, 2)
+        case 3: FQ_PAIR(float, 3)
+        default: return hipErrorInvalidValue;
+    }
+#undef FQ_PAIR
+}
+
+hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, const TileArgs &a,
+                          double *partials, hipStream_t stream)
+{
+    const uint64_t tiles = 1ull << a.nh;
+    const dim3 g((unsigned)tiles), b(1u << a.nthr);
+#define FQ_CM(R, AL)                                                                                       \
+    hipLaunchKernelGGL((k_cut_mul<R, AL>), g, b, 0, stream,                                               \
+                       reinterpret_cast<const typename CTA<R>::T *>(psi),                                 \
+                       reinterpret_cast<typename CTA<R>::T *>(lam), a, partials);                         \
+    return hipGetLastError();
+    if (dtype == 0) {
+        switch (a_log) {
+            case 1: { FQ_CM(double, 1) }
+            case 2: { FQ_CM(double, 2) }
+            case 3: { FQ_CM(double, 3) }
+            case 4: { FQ_CM(double, 4) }
+            default: return hipErrorInvalidValue;
+        }
+    }
+    switch (a_log) {
+        case 1: { FQ_CM(float, 1) }
+        case 2: { FQ_CM(float, 2) }
+        case 3: { FQ_CM(float, 3) }
+        case 4: { FQ_CM(float, 4) }
+        default: return hipErrorInvalidValue;
+    }
+#undef FQ_CM
+}
+
+}  // namespace fq

@@ -73,6 +73,9 @@ struct fq_state {
     int cand_blocks = 0;
     double2 *d_phase = nullptr;  // per-launch cost phase factors
     size_t phase_cap = 0;        // entries
+    void *aux = nullptr;         // adjoint state (lambda), allocated on first use
+    double *d_grad = nullptr;    // adjoint per-pass sums
+    size_t grad_cap = 0;
     bool timing = false;
     std::vector<TimingRec> pending;
     std::vector<hipEvent_t> pool;
@@ -506,6 +509,8 @@ extern "C" int fq_state_destroy(fq_state *st)
     hipFree(st->d_cand_p);
     hipFree(st->d_cand_i);
     hipFree(st->d_phase);
+    hipFree(st->aux);
+    hipFree(st->d_grad);
     if (st->own)
         hipStreamDestroy(st->own);
     delete st;
@@ -750,6 +755,44 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
 
 static int carriers_for(int dtype) { return dtype == FQ_C128 ? 3 : 4; }
 
+// Tile passes covering every qubit once: a contiguous low tile of Qmax
+// qubits, then the rest spread evenly over strided tiles that carry the
+// lowest qubits (at least 128 contiguous bytes per run).
+struct FusedPass {
+    std::vector<int> tile, gates;
+};
+
+static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
+{
+    std::vector<FusedPass> ps;
+    Qmax = std::min(Qmax, N);
+    FusedPass p1;
+    p1.tile = low_tile(Qmax);
+    p1.gates = p1.tile;
+    ps.push_back(p1);
+    const int rest = N - Qmax;
+    if (rest <= 0)
+        return ps;
+    const int cmin = std::min(carriers_for(dtype), Qmax - 1);
+    const int per_max = Qmax - cmin;
+    const int npass = (rest + per_max - 1) / per_max;
+    int next = Qmax;
+    for (int k = 0; k < npass; ++k) {
+        const int n_new = rest / npass + (k < rest % npass ? 1 : 0);
+        const int c = std::min(Qmax - n_new, next);
+        FusedPass p;
+        for (int i = 0; i < c; ++i)
+            p.tile.push_back(i);
+        for (int i = 0; i < n_new; ++i) {
+            p.tile.push_back(next + i);
+            p.gates.push_back(next + i);
+        }
+        next += n_new;
+        ps.push_back(p);
+    }
+    return ps;
+}
+
 // Split arbitrary RX targets into tile passes (targets below C: one chunk pass).
 static int rx_targets(fq_state *st, std::vector<int> targets, double beta)
 {
@@ -860,44 +903,11 @@ extern "C" int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_sw
 
 // ---- schedules -----------------------------------------------------------------------
 
-struct FusedPass {
-    std::vector<int> tile, gates;
-};
-
-// Tile passes of one fused layer.  Pass 1 owns the lowest Qmax qubits
-// (contiguous tile, cost phase fused).  The remaining qubits are spread evenly
-// over the fewest strided passes; every strided tile also carries the lowest
-// qubits (at least 128 contiguous bytes per run) so that its HBM accesses
-// stay coalesced, and a pass with fewer new qubits carries more.
+// Tile passes of one fused layer: pass 1 owns the lowest Qmax qubits
+// (contiguous tile, cost phase fused), the others are strided.
 static std::vector<FusedPass> fused_passes(int N, int dtype)
 {
-    std::vector<FusedPass> ps;
-    const int Qmax = std::min(tile_max_q(dtype), N);
-    FusedPass p1;
-    p1.tile = low_tile(Qmax);
-    p1.gates = p1.tile;
-    ps.push_back(p1);
-    const int rest = N - Qmax;
-    if (rest <= 0)
-        return ps;
-    const int cmin = std::min(carriers_for(dtype), Qmax - 1);
-    const int per_max = Qmax - cmin;
-    const int npass = (rest + per_max - 1) / per_max;
-    int next = Qmax;
-    for (int k = 0; k < npass; ++k) {
-        const int n_new = rest / npass + (k < rest % npass ? 1 : 0);
-        const int c = std::min(Qmax - n_new, next);
-        FusedPass p;
-        for (int i = 0; i < c; ++i)
-            p.tile.push_back(i);
-        for (int i = 0; i < n_new; ++i) {
-            p.tile.push_back(next + i);
-            p.gates.push_back(next + i);
-        }
-        next += n_new;
-        ps.push_back(p);
-    }
-    return ps;
+    return cover_passes(N, tile_max_q(dtype), dtype);
 }
 
 extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
@@ -981,6 +991,122 @@ extern "C" int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *
     *expect = h[0];
     if (norm2)
         *norm2 = h[1];
+    return FQ_OK;
+}
+
+// ---- adjoint gradient ------------------------------------------------------------------
+
+static int ensure_aux(fq_state *st, size_t grad_slots)
+{
+    if (!st->aux) {
+        if (hipMalloc(&st->aux, st->bytes) != hipSuccess) {
+            hipGetLastError();
+            st->aux = nullptr;
+            return set_err(FQ_ERR_OOM, "adjoint state allocation of %zu bytes failed", st->bytes);
+        }
+    }
+    if (grad_slots > st->grad_cap) {
+        FQ_HIP(hipStreamSynchronize(st->stream));
+        hipFree(st->d_grad);
+        st->d_grad = nullptr;
+        st->grad_cap = 0;
+        if (hipMalloc(&st->d_grad, sizeof(double) * 2 * grad_slots) != hipSuccess)
+            return set_err(FQ_ERR_OOM, "gradient scratch allocation failed");
+        st->grad_cap = grad_slots;
+    }
+    return FQ_OK;
+}
+
+// Reverse-uncompute gradient of <C> (fq_adjoint.hip).  The state must hold
+// psi_P of fq_run(FUSED) with the same graph and angles (identity qubit map);
+// it is consumed (walked back to |+>^N).
+extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gammas,
+                           const double *betas, double *expect, double *grad_gammas,
+                           double *grad_betas)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (int rc = graph_matches(st, g))
+        return rc;
+    if (P < 1 || !gammas || !betas || !grad_gammas || !grad_betas)
+        return set_err(FQ_ERR_ARG, "bad layer count or null arrays");
+    for (int p = 0; p < st->N; ++p)
+        if (st->qmap[p] != p)
+            return set_err(FQ_ERR_ARG, "fq_backward needs the identity qubit map (fused schedule)");
+    const int N = st->N;
+    // 2^12-amplitude tiles of psi and lambda share the LDS (2 x 68 KiB);
+    // 512 threads x 8 amplitudes of each state
+    const int Qb = std::min(12, N);
+    const std::vector<FusedPass> ps = cover_passes(N, Qb, st->dtype);
+    const size_t npass = ps.size();
+    if (int rc = ensure_aux(st, 1 + npass * (size_t)P))
+        return rc;
+
+    // lambda = Cut * psi_P, <C> into slot 0
+    {
+        const int Q = std::min(12, N);
+        const int a_log = std::max(1, std::min(4, Q));
+        Geom geom = make_geom(N, low_tile(Q), a_log);
+        TileArgs a;
+        geom_to_args(geom, a);
+        if (int rc = get_plan(g, geom, &a.plan))
+            return rc;
+        FQ_HIP(launch_cut_mul(st->dtype, a_log, st->sv, st->aux, a, st->d_part, st->stream));
+        FQ_HIP(launch_finalize(st->d_part, 1ull << a.nh, st->d_grad, st->stream));
+    }
+    for (int k = P - 1; k >= 0; --k) {
+        for (int i = (int)npass - 1; i >= 0; --i) {
+            const bool costbw = i == 0;
+            const int Q = (int)ps[i].tile.size();
+            const int a_log = std::max(1, std::min(3, Q));
+            Geom geom = make_geom(N, ps[i].tile, a_log);
+            if (geom_runs(geom) > FQ_MAX_RUNS)
+                return set_err(FQ_ERR_ARG, "internal: adjoint tile needs too many bit runs");
+            TileArgs a;
+            geom_to_args(geom, a);
+            uint32_t gm = 0;
+            for (int q : ps[i].gates)
+                for (int ti = 0; ti < Q; ++ti)
+                    if (ps[i].tile[ti] == q)
+                        gm |= 1u << ti;
+            if (int rc = set_gates(a, geom, gm))
+                return rc;
+            double sn, cs;
+            sincos(-betas[k] / 2, &sn, &cs);
+            a.rx_c = cs;
+            a.rx_s = sn;
+            if (costbw) {
+                if (int rc = get_plan(g, geom, &a.plan))
+                    return rc;
+                set_cost(a, -gammas[k], st->init_amp);
+                if (int rc = prepare_phases(st, a, a_log))
+                    return rc;
+            }
+            double *slot = st->d_grad + 2 * (1 + (size_t)k * npass + i);
+            int rc = timed(st, FQ_K_ADJOINT, 4.0 * (double)st->bytes, [&] {
+                hipError_t e = launch_pair(st->dtype, a_log, costbw, st->sv, st->aux, a,
+                                           costbw ? st->d_phase : nullptr, st->d_part, st->stream);
+                if (e != hipSuccess)
+                    return e;
+                return launch_finalize(st->d_part, pair_grid(a), slot, st->stream);
+            });
+            if (rc)
+                return rc;
+        }
+    }
+    std::vector<double> h(2 * (1 + npass * (size_t)P));
+    FQ_HIP(hipMemcpyAsync(h.data(), st->d_grad, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                          st->stream));
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    if (expect)
+        *expect = h[0];
+    for (int k = 0; k < P; ++k) {
+        double gb = 0.0;
+        for (size_t i = 0; i < npass; ++i)
+            gb += h[2 * (1 + k * npass + i)];
+        grad_betas[k] = gb;
+        grad_gammas[k] = -h[2 * (1 + k * npass) + 1];
+    }
     return FQ_OK;
 }
 

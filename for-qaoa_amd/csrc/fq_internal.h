@@ -67,6 +67,20 @@ struct TileArgs {
     PlanDev plan;
 };
 
+// Per-slot parameters of a packed (slab) step, in device memory.
+struct SlotParams {
+    double rx_c, rx_s;       // RX(beta) of the slot's current layer
+    double gamma;            // cost angle of the slot's current layer
+    double init_amp;         // (1/sqrt 2)^N
+    int active;              // the slot runs this step
+    int cost_first;          // layer 0: launch-control write, no read
+    int has_rr;
+    int want_expect;         // the slot finishes this step: reduce <C>
+    const double2 *ptab;     // the slot's phase table for this step
+    PlanDev plan;            // cut plan of the first (contiguous) tile pass
+    PlanDev eplan;           // cut plan of the <C> reduction geometry
+};
+
 struct SwapArgs {
     int N, out, in, k;
     int t;                   // tiled: sub-tile log2 side
@@ -90,6 +104,21 @@ hipError_t launch_topk(int dtype, const void *sv, int N, int nblocks, double *ca
 hipError_t launch_gather(int dtype, const void *sv, const uint64_t *idx, size_t n, double *out,
                          hipStream_t stream);
 hipError_t launch_fill_zero_one(int dtype, void *sv, int N, hipStream_t stream);
+hipError_t launch_finalize(const double *partials, uint64_t n, double *out2, hipStream_t stream);
+// adjoint (fq_adjoint.hip)
+hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
+                       const double2 *ptab, double *partials, hipStream_t stream);
+hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, const TileArgs &a,
+                          double *partials, hipStream_t stream);
+unsigned pair_grid(const TileArgs &a);
+size_t pair_lds_bytes(int dtype, int Q);
+// packed slabs (fq_serve.hip)
+hipError_t launch_slab_phase_table(const TileArgs &local, int a_log, const SlotParams *slots, int nslots,
+                                   double2 *tabs, size_t stride, hipStream_t stream);
+hipError_t launch_slab_pass(int dtype, int a_log, bool cost, void *sv, const TileArgs &a,
+                            const SlotParams *slots, int slot_shift, hipStream_t stream);
+hipError_t launch_slab_expect(int dtype, int a_log, const void *sv, const TileArgs &a, const SlotParams *slots,
+                              int slot_shift, int nslots, double *partials, double *out, hipStream_t stream);
 int tile_max_q(int dtype);
 size_t tile_lds_bytes(int dtype, int Q);
 

@@ -825,6 +825,12 @@ static hipError_t launch_expect_t(const void *sv, const TileArgs &a, double *par
     return hipGetLastError();
 }
 
+hipError_t launch_finalize(const double *partials, uint64_t n, double *out2, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream, partials, n, out2);
+    return hipGetLastError();
+}
+
 hipError_t launch_expect(int dtype, int a_log, const void *sv, const TileArgs &a, double *partials,
                          double *out2, hipStream_t stream)
 {

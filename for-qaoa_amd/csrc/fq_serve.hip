@@ -1,0 +1,496 @@
+// fq_serve.hip -- packed execution of many QAOA evaluations of one size N.
+//
+// No reference counterpart (the reference runs one evaluation per process).
+// A slab holds K = 2^s state slots of 2^N amplitudes in one allocation; seen
+// as one virtual state of N + s qubits whose top s bits select the slot, a
+// single persistent tile-pass launch covers every active slot.  Each slot has
+// its own graph, angles and current layer (per-slot parameters in device
+// memory), so requests of different depth P advance together one layer per
+// step and a request can enter a free slot at any layer boundary.
+#include "forqaoa.h"
+#include "fq_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace fq {
+
+template <typename R> struct CTS;
+template <> struct CTS<double> { typedef double2 T; };
+template <> struct CTS<float> { typedef float2 T; };
+
+__device__ __forceinline__ int lds_pad_s(int e) { return e + (e >> 4); }
+
+__device__ __forceinline__ void lds_barrier_s()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ uint64_t unpack_runs_s(uint64_t x, const BitRuns &r)
+{
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
+        if (k < r.n) {
+            const uint32_t d = r.run[k];
+            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
+        }
+    }
+    return o;
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ void rx_pair_s(T &q0, T &q1, R c, R s)
+{
+    T o0, o1;
+    o0.x = q0.x * c + q1.y * s;
+    o0.y = q0.y * c - q1.x * s;
+    o1.x = q0.y * s + q1.x * c;
+    o1.y = -q0.x * s + q1.y * c;
+    q0 = o0;
+    q1 = o1;
+}
+
+template <typename T, typename R, int A_LOG>
+__device__ __forceinline__ void rx_regs_s(T *v, unsigned mask, R c, R s)
+{
+    constexpr int A = 1 << A_LOG;
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r)
+        if ((mask >> r) & 1u) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                if (!((j >> r) & 1))
+                    rx_pair_s<T, R>(v[j], v[j | (1 << r)], c, s);
+        }
+}
+
+template <typename T, typename R>
+__device__ __forceinline__ void cmul_s(T &v, R c, R s)
+{
+    const R x = v.x, y = v.y;
+    v.x = x * c - y * s;
+    v.y = x * s + y * c;
+}
+
+// Phase tables of every active slot (layout of k_phase_table per slot, at
+// slot * stride).  Grid: (entries, slots).
+__global__ void __launch_bounds__(256) k_slab_phase_table(const TileArgs a, int a_log,
+                                                          const SlotParams *__restrict__ slots,
+                                                          double2 *__restrict__ tabs, size_t stride)
+{
+    const SlotParams &sp = slots[blockIdx.y];
+    if (!sp.active)
+        return;
+    const PlanDev &p = sp.plan;
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t q1 = (uint64_t)a.Q + 1;
+    const uint64_t ntile_e = (1ull << a.nh) * q1;
+    const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
+    double angle;
+    if (gid < ntile_e) {
+        const uint64_t tile = gid / q1;
+        const int i = (int)(gid - tile * q1);
+        if (i < a.Q) {
+            double acc = 0.0;
+            for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
+                acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
+            angle = sp.gamma * (p.cH[i] - 2.0 * acc);
+        } else {
+            double K = 0.0;
+            for (int e = 0; e < p.n_hh; ++e)
+                K += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
+            for (int h = 0; h < a.nh; ++h)
+                K += p.omega[h] * (double)((tile >> h) & 1ull);
+            angle = sp.gamma * (K - 0.5 * p.wtot);
+        }
+    } else if (gid < ntile_e + nthr_e) {
+        angle = sp.gamma * p.thr[gid - ntile_e];
+    } else if (gid < ntile_e + nthr_e + 16) {
+        angle = sp.gamma * p.rr[gid - ntile_e - nthr_e];
+    } else {
+        return;
+    }
+    double sn, cs;
+    sincos(angle, &sn, &cs);
+    tabs[stride * blockIdx.y + gid] = make_double2(cs, sn);
+}
+
+// One tile pass over the slab.  a describes the virtual state (N + s qubits;
+// the slot bits are the top high bits); tile index >> slot_shift is the slot.
+template <typename R, int A_LOG, bool COST>
+__global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restrict__ sv, const TileArgs a,
+                                                    const SlotParams *__restrict__ slots, int slot_shift)
+{
+    typedef typename CTS<R>::T T;
+    constexpr int A = 1 << A_LOG;
+    extern __shared__ __align__(16) unsigned char fq_smem_s[];
+    const int Q = a.Q, nthr = a.nthr;
+    const int t = threadIdx.x;
+    const uint64_t ntiles = 1ull << a.nh;
+    const uint64_t local_mask = (1ull << slot_shift) - 1ull;
+    const uint64_t toff = unpack_runs_s((uint64_t)t, a.truns);
+    uint64_t roff[A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                o |= 1ull << a.rpos[r];
+        roff[j] = o;
+    }
+    const int qmask = (1 << Q) - 1;
+    const uint64_t q1 = (uint64_t)Q + 1;
+    T *lds = reinterpret_cast<T *>(fq_smem_s);
+    const int nat = lds_pad_s(t);
+
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const SlotParams &sp = slots[tile >> slot_shift];
+        if (!sp.active)
+            continue;  // uniform over the work-group
+        const uint64_t lt = tile & local_mask;
+        T *const p0 = sv + (unpack_runs_s(tile, a.hruns) + toff);
+        T v[A];
+        const bool first = COST && sp.cost_first;
+        if (!first) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                v[j] = p0[roff[j]];
+        }
+        if (COST) {
+            const double2 *f_t = sp.ptab + lt * q1;
+            const double2 *thr_tab = sp.ptab + (1ull << slot_shift) * q1;
+            const double2 *rr_tab = thr_tab + ((size_t)(1 + A_LOG) << nthr);
+            const double2 f_thr = thr_tab[t];
+            double x = f_thr.x, y = f_thr.y;
+            {
+                const double2 f = f_t[Q];
+                const double nx = x * f.x - y * f.y;
+                y = x * f.y + y * f.x;
+                x = nx;
+            }
+            for (int i = 0; i < nthr; ++i) {
+                const double2 f = f_t[i];
+                const double nx = x * f.x - y * f.y;
+                const double ny = x * f.y + y * f.x;
+                const bool on = (t >> i) & 1;
+                x = on ? nx : x;
+                y = on ? ny : y;
+            }
+            if (first) {
+                x *= sp.init_amp;
+                y *= sp.init_amp;
+            }
+            const R c0 = (R)x, s0 = (R)y;
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                if (first) {
+                    v[j].x = c0;
+                    v[j].y = s0;
+                } else {
+                    cmul_s(v[j], c0, s0);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < A_LOG; ++r) {
+                const double2 fr = thr_tab[((1 + r) << nthr) + t];
+                const double2 fq = f_t[nthr + r];
+                const R fx = (R)(fr.x * fq.x - fr.y * fq.y);
+                const R fy = (R)(fr.x * fq.y + fr.y * fq.x);
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    if ((j >> r) & 1)
+                        cmul_s(v[j], fx, fy);
+            }
+            if (sp.has_rr) {
+#pragma unroll
+                for (int j = 0; j < A; ++j) {
+                    const double2 f = rr_tab[j];
+                    cmul_s(v[j], (R)f.x, (R)f.y);
+                }
+            }
+        }
+        const R c = (R)sp.rx_c, s = (R)sp.rx_s;
+        rx_regs_s<T, R, A_LOG>(v, a.gate_mask >> nthr, c, s);
+        if (a.ngroups > 0) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                lds[nat + lds_pad_s(j << nthr)] = v[j];
+            lds_barrier_s();
+            for (int g = 0; g < a.ngroups; ++g) {
+                const unsigned gm = a.grp_mask[g];
+                const unsigned rest = (unsigned)qmask & ~gm;
+                int et = 0;
+                {
+                    int k = 0;
+                    for (int i = 0; i < Q; ++i)
+                        if ((rest >> i) & 1u) {
+                            et |= ((t >> k) & 1) << i;
+                            ++k;
+                        }
+                }
+                const int etp = lds_pad_s(et);
+                int pr[A_LOG > 0 ? A_LOG : 1];
+                unsigned g2 = gm;
+#pragma unroll
+                for (int r = 0; r < A_LOG; ++r) {
+                    pr[r] = __builtin_ctz(g2);
+                    g2 &= g2 - 1u;
+                }
+                int ej[A];
+#pragma unroll
+                for (int j = 0; j < A; ++j) {
+                    int e = 0;
+#pragma unroll
+                    for (int r = 0; r < A_LOG; ++r)
+                        if ((j >> r) & 1)
+                            e |= 1 << pr[r];
+                    ej[j] = lds_pad_s(e);
+                }
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    v[j] = lds[etp + ej[j]];
+                rx_regs_s<T, R, A_LOG>(v, a.grp_gates[g], c, s);
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    lds[etp + ej[j]] = v[j];
+                lds_barrier_s();
+            }
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                v[j] = lds[nat + lds_pad_s(j << nthr)];
+        }
+#pragma unroll
+        for (int j = 0; j < A; ++j)
+            p0[roff[j]] = v[j];
+    }
+}
+
+// <C> and sum |psi|^2 of the slots with want_expect; per-tile partials.
+template <typename R, int A_LOG>
+__global__ void __launch_bounds__(256) k_slab_expect(const typename CTS<R>::T *__restrict__ sv,
+                                                     const TileArgs a,
+                                                     const SlotParams *__restrict__ slots,
+                                                     int slot_shift, double *__restrict__ partials)
+{
+    typedef typename CTS<R>::T T;
+    constexpr int A = 1 << A_LOG;
+    __shared__ double s_c[FQ_MAXQ + 2];
+    __shared__ double s_K;
+    __shared__ double s_pc[256];
+    __shared__ double s_p[256];
+    const int t = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const uint64_t tile = blockIdx.x;
+    const SlotParams &sp = slots[tile >> slot_shift];
+    if (!sp.want_expect) {
+        if (t == 0) {
+            partials[2 * tile] = 0.0;
+            partials[2 * tile + 1] = 0.0;
+        }
+        return;
+    }
+    const PlanDev &p = sp.eplan;
+    const uint64_t lt = tile & ((1ull << slot_shift) - 1ull);
+    const uint64_t base = unpack_runs_s(tile, a.hruns) + unpack_runs_s((uint64_t)t, a.truns);
+    for (int i = t; i < a.Q; i += nthreads) {
+        double acc = 0.0;
+        for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
+            acc += p.csr_w[e] * (double)((lt >> p.csr_h[e]) & 1ull);
+        s_c[i] = p.cH[i] - 2.0 * acc;
+    }
+    if (t == 0) {
+        double K = 0.0;
+        for (int e = 0; e < p.n_hh; ++e)
+            K += p.hh_w[e] * (double)(((lt >> p.hh_u[e]) ^ (lt >> p.hh_v[e])) & 1ull);
+        for (int h = 0; h < slot_shift; ++h)
+            K += p.omega[h] * (double)((lt >> h) & 1ull);
+        s_K = K;
+    }
+    __syncthreads();
+    double cbase = s_K + p.thr[t];
+    for (int i = 0; i < a.nthr; ++i)
+        cbase += s_c[i] * (double)((t >> i) & 1);
+    double cr[A_LOG > 0 ? A_LOG : 1];
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r)
+        cr[r] = s_c[a.nthr + r] + p.thr[((1 + r) << a.nthr) + t];
+    double acc_pc = 0.0, acc_p = 0.0;
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                o |= 1ull << a.rpos[r];
+        double cut = cbase + p.rr[j];
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((j >> r) & 1)
+                cut += cr[r];
+        const T v = sv[base + o];
+        const double re = (double)v.x, im = (double)v.y;
+        const double pr = re * re + im * im;
+        acc_pc += pr * cut;
+        acc_p += pr;
+    }
+    s_pc[t] = acc_pc;
+    s_p[t] = acc_p;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && t + w < nthreads) {
+            s_pc[t] += s_pc[t + w];
+            s_p[t] += s_p[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        partials[2 * tile] = s_pc[0];
+        partials[2 * tile + 1] = s_p[0];
+    }
+}
+
+// One work-group per slot: sum that slot's tile partials in a fixed order.
+__global__ void __launch_bounds__(256) k_slab_finalize(const double *__restrict__ partials,
+                                                       uint64_t tiles_per_slot, double *__restrict__ out)
+{
+    __shared__ double s_a[256];
+    __shared__ double s_b[256];
+    const int t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * tiles_per_slot;
+    double x = 0.0, y = 0.0;
+    for (uint64_t i = t; i < tiles_per_slot; i += 256) {
+        x += partials[2 * (base + i)];
+        y += partials[2 * (base + i) + 1];
+    }
+    s_a[t] = x;
+    s_b[t] = y;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s_a[t] += s_a[t + w];
+            s_b[t] += s_b[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[2 * blockIdx.x] = s_a[0];
+        out[2 * blockIdx.x + 1] = s_b[0];
+    }
+}
+
+// ---- launchers -----------------------------------------------------------------
+
+static int cus_s()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+hipError_t launch_slab_phase_table(const TileArgs &local, int a_log, const SlotParams *slots, int nslots,
+                                   double2 *tabs, size_t stride, hipStream_t stream)
+{
+    const size_t n = phase_table_entries(local, a_log);
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)nslots);
+    hipLaunchKernelGGL(k_slab_phase_table, grid, dim3(256), 0, stream, local, a_log, slots, tabs, stride);
+    return hipGetLastError();
+}
+
+template <typename R, int A_LOG, bool COST>
+static hipError_t launch_slab_pass_t(void *sv, const TileArgs &a, const SlotParams *slots, int slot_shift,
+                                     hipStream_t stream)
+{
+    typedef typename CTS<R>::T T;
+    const size_t lds = a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
+    const void *fn = reinterpret_cast<const void *>(&k_slab_pass<R, A_LOG, COST>);
+    if (lds > 65536) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024 - 1024);
+            if (e != hipSuccess)
+                return e;
+            attr_set = true;
+        }
+    }
+    const unsigned threads = 1u << a.nthr;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint64_t tiles = 1ull << a.nh;
+    const uint64_t resident = (uint64_t)per_cu * cus_s();
+    const uint64_t grid = tiles < resident ? tiles : resident;
+    hipLaunchKernelGGL((k_slab_pass<R, A_LOG, COST>), dim3((unsigned)grid), dim3(threads), lds, stream,
+                       reinterpret_cast<T *>(sv), a, slots, slot_shift);
+    return hipGetLastError();
+}
+
+hipError_t launch_slab_pass(int dtype, int a_log, bool cost, void *sv, const TileArgs &a,
+                            const SlotParams *slots, int slot_shift, hipStream_t stream)
+{
+#define FQ_SP(R, AL) \
+    return cost ? launch_slab_pass_t<R, AL, true>(sv, a, slots, slot_shift, stream) \
+                : launch_slab_pass_t<R, AL, false>(sv, a, slots, slot_shift, stream);
+    if (dtype == 0) {
+        switch (a_log) {
+            case 1: FQ_SP(double, 1)
+            case 2: FQ_SP(double, 2)
+            case 3: FQ_SP(double, 3)
+            default: return hipErrorInvalidValue;
+        }
+    }
+    switch (a_log) {
+        case 1: FQ_SP(float, 1)
+        case 2: FQ_SP(float, 2)
+        case 3: FQ_SP(float, 3)
+        case 4: FQ_SP(float, 4)
+        default: return hipErrorInvalidValue;
+    }
+#undef FQ_SP
+}
+
+hipError_t launch_slab_expect(int dtype, int a_log, const void *sv, const TileArgs &a, const SlotParams *slots,
+                              int slot_shift, int nslots, double *partials, double *out, hipStream_t stream)
+{
+    const uint64_t tiles = 1ull << a.nh;
+    const dim3 g((unsigned)tiles), b(1u << a.nthr);
+#define FQ_SE(R, AL)                                                                                  \
+    hipLaunchKernelGGL((k_slab_expect<R, AL>), g, b, 0, stream,                                      \
+                       reinterpret_cast<const typename CTS<R>::T *>(sv), a, slots, slot_shift, partials); \
+    break;
+    if (dtype == 0) {
+        switch (a_log) {
+            case 1: FQ_SE(double, 1)
+            case 2: FQ_SE(double, 2)
+            case 3: FQ_SE(double, 3)
+            case 4: FQ_SE(double, 4)
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (a_log) {
+            case 1: FQ_SE(float, 1)
+            case 2: FQ_SE(float, 2)
+            case 3: FQ_SE(float, 3)
+            case 4: FQ_SE(float, 4)
+            default: return hipErrorInvalidValue;
+        }
+    }
+#undef FQ_SE
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_slab_finalize, dim3(nslots), dim3(256), 0, stream, partials,
+                       tiles / (uint64_t)nslots, out);
+    return hipGetLastError();
+}
+
+}  // namespace fq

@@ -173,6 +173,19 @@ class State:
             raise FQError(_lib.FQ_ERR_ARG, "gammas and betas differ in length")
         check(lib().fq_run(self.handle, graph.handle, len(g), _dptr(g), _dptr(b), SCHEDULES[schedule]))
 
+    def backward(self, graph, gammas, betas):
+        """Adjoint gradient of <C> after run(graph, gammas, betas, "fused").
+
+        Consumes the state.  Returns (<C>, dC/dgammas, dC/dbetas)."""
+        g = np.ascontiguousarray(gammas, dtype=np.float64)
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        gg = np.zeros(len(g))
+        gb = np.zeros(len(b))
+        e = ctypes.c_double()
+        check(lib().fq_backward(self.handle, graph.handle, len(g), _dptr(g), _dptr(b), ctypes.byref(e),
+                                _dptr(gg), _dptr(gb)))
+        return e.value, gg, gb
+
     def expect_cut(self, graph):
         e = ctypes.c_double()
         n = ctypes.c_double()

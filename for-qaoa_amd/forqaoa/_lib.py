@@ -82,6 +82,7 @@ SIGNATURES = {
     "fq_state_copy_in": (_i, [_vp, _vp, _sz]),
     "fq_state_gather": (_i, [_vp, _up, _sz, _dp, _i]),
     "fq_sync": (_i, [_vp]),
+    "fq_backward": (_i, [_vp, _vp, _i, _dp, _dp, _dp, _dp, _dp]),
     "fq_timing_enable": (_i, [_vp, _i]),
     "fq_timing_get": (_i, [_vp, _i, _dp, _i64p, _dp]),
     "fq_timing_reset": (_i, [_vp]),

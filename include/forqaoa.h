@@ -151,6 +151,18 @@ int fq_state_gather(fq_state *st, const uint64_t *idx, size_t n, double *out, in
 /* Wait for all queued work of the state. */
 int fq_sync(fq_state *st);
 
+/* ---- adjoint gradient (no reference counterpart: the reference computes no
+ *      expectation and no gradient) --------------------------------------- */
+
+/* d<C>/dgamma_k and d<C>/dbeta_k of the P-layer evaluation by reverse
+ * uncompute (no checkpoints): st must hold the state fq_run(st, g, P, gammas,
+ * betas, FQ_SCHED_FUSED) produced (identity qubit map).  The state is consumed
+ * (walked back to the launch-control state); a second state-sized buffer is
+ * allocated on first use.  expect (may be NULL) receives <C>.  FP64
+ * accumulation, fixed reduction order.  Waits. */
+int fq_backward(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
+                double *expect, double *grad_gammas, double *grad_betas);
+
 /* ---- timing (src/rx.cu:139-157 accumulates rx_gate_time_ms the same way) */
 
 /* Enable/disable per-launch HIP event timing on the state's stream. */
