@@ -37,6 +37,44 @@ static int set_err(int code, const char *fmt, ...)
     } while (0)
 
 extern "C" const char *fq_last_error(void) { return g_err; }
+
+// ---- HBM budget (states and slabs) ----------------------------------------------
+
+#include <mutex>
+static std::mutex g_budget_mu;
+static size_t g_budget = SIZE_MAX;
+static size_t g_in_use = 0;
+
+static bool budget_take(size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_budget_mu);
+    if (g_in_use + bytes > g_budget || g_in_use + bytes < g_in_use)
+        return false;
+    g_in_use += bytes;
+    return true;
+}
+
+static void budget_give(size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_budget_mu);
+    g_in_use = bytes > g_in_use ? 0 : g_in_use - bytes;
+}
+
+extern "C" int fq_set_memory_budget(size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_budget_mu);
+    g_budget = bytes ? bytes : SIZE_MAX;
+    return FQ_OK;
+}
+
+extern "C" int fq_memory_in_use(size_t *bytes)
+{
+    if (!bytes)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    std::lock_guard<std::mutex> lk(g_budget_mu);
+    *bytes = g_in_use;
+    return FQ_OK;
+}
 extern "C" int fq_version(void) { return 100; }
 
 // ---- handles -------------------------------------------------------------------
@@ -50,6 +88,9 @@ struct fq_graph {
     int N = 0;
     std::vector<double> W;  // N*N, W[min*N+max], current labels
     std::map<std::string, PlanEntry> plans;
+    uint64_t version = 1;       // bumped by every change of W
+    uint64_t hashed_version = 0;
+    uint64_t w_hash = 0;        // content hash of W at hashed_version
 };
 
 struct TimingRec {
@@ -275,7 +316,11 @@ static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 static int get_plan(fq_graph *G, const Geom &g, PlanDev *out)
 {
     const int N = G->N;
-    const uint64_t wh = hash_bytes(G->W.data(), G->W.size() * sizeof(double));
+    if (G->hashed_version != G->version) {
+        G->w_hash = hash_bytes(G->W.data(), G->W.size() * sizeof(double));
+        G->hashed_version = G->version;
+    }
+    const uint64_t wh = G->w_hash;
     std::string key = geom_key(g);
     key.append(reinterpret_cast<const char *>(&wh), sizeof(wh));
     auto it = G->plans.find(key);
@@ -468,8 +513,14 @@ extern "C" int fq_state_create(int N, int D, int B, int C, int dtype, fq_state *
     for (int p = 0; p < FQ_MAXN; ++p)
         st->qmap[p] = p;
     st->init_amp = pow(1. / sqrt(2.), N);
+    if (!budget_take(st->bytes)) {
+        delete st;
+        return set_err(FQ_ERR_BUDGET, "state of %zu bytes exceeds the memory budget",
+                       ((size_t)1 << N) * (dtype == FQ_C128 ? 16 : 8));
+    }
     if (hipMalloc(&st->sv, st->bytes) != hipSuccess) {
         hipGetLastError();
+        budget_give(((size_t)1 << N) * (dtype == FQ_C128 ? 16 : 8));
         delete st;
         return set_err(FQ_ERR_OOM, "state allocation of %zu bytes failed", ((size_t)1 << N) * (dtype == FQ_C128 ? 16 : 8));
     }
@@ -503,12 +554,16 @@ extern "C" int fq_state_destroy(fq_state *st)
     }
     for (auto e : st->pool)
         hipEventDestroy(e);
+    if (st->sv)
+        budget_give(st->bytes);
     hipFree(st->sv);
     hipFree(st->d_part);
     hipFree(st->d_out2);
     hipFree(st->d_cand_p);
     hipFree(st->d_cand_i);
     hipFree(st->d_phase);
+    if (st->aux)
+        budget_give(st->bytes);
     hipFree(st->aux);
     hipFree(st->d_grad);
     if (st->own)
@@ -598,6 +653,7 @@ extern "C" int fq_graph_add_edge(fq_graph *g, int u, int v, double w)
     if (u > v)
         std::swap(u, v);
     g->W[(size_t)u * g->N + v] = w;
+    g->version++;
     return FQ_OK;
 }
 
@@ -642,6 +698,7 @@ extern "C" int fq_graph_relabel(fq_graph *g, int swap_out, int swap_in, int num_
             std::swap(wa, wb);
         }
     }
+    g->version++;
     return FQ_OK;
 }
 
@@ -999,9 +1056,12 @@ extern "C" int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *
 static int ensure_aux(fq_state *st, size_t grad_slots)
 {
     if (!st->aux) {
+        if (!budget_take(st->bytes))
+            return set_err(FQ_ERR_BUDGET, "adjoint state of %zu bytes exceeds the memory budget", st->bytes);
         if (hipMalloc(&st->aux, st->bytes) != hipSuccess) {
             hipGetLastError();
             st->aux = nullptr;
+            budget_give(st->bytes);
             return set_err(FQ_ERR_OOM, "adjoint state allocation of %zu bytes failed", st->bytes);
         }
     }
@@ -1283,5 +1343,327 @@ extern "C" int fq_timing_reset(fq_state *st)
         st->t_cnt[k] = 0;
         st->t_bytes[k] = 0;
     }
+    return FQ_OK;
+}
+
+// ---- packed slabs (continuous batching) ------------------------------------------------
+
+struct SlotHost {
+    fq_graph *g = nullptr;
+    int P = 0, layer = 0;
+    int phase = 0;  // 0 free, 1 active, 2 finished (result pending collection)
+    std::vector<double> gammas, betas;
+};
+
+struct fq_slab {
+    int N = 0, K = 0, s = 0, dtype = 0;
+    size_t elem = 8, slot_bytes = 0, bytes = 0;
+    void *sv = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<SlotHost> slots;
+    SlotParams *d_params = nullptr, *h_params = nullptr;
+    double2 *d_tabs = nullptr;
+    size_t tab_stride = 0;
+    double *d_part = nullptr;
+    double *d_res = nullptr, *h_res = nullptr;
+    hipEvent_t done = nullptr;
+    bool pending = false;  // a step was enqueued and not yet observed complete
+    double init_amp = 0;
+    std::vector<FusedPass> passes;
+};
+
+static void slab_free(fq_slab *sb)
+{
+    if (sb->stream)
+        hipStreamSynchronize(sb->stream);
+    if (sb->sv)
+        budget_give(sb->bytes);
+    hipFree(sb->sv);
+    hipFree(sb->d_params);
+    hipHostFree(sb->h_params);
+    hipFree(sb->d_tabs);
+    hipFree(sb->d_part);
+    hipFree(sb->d_res);
+    hipHostFree(sb->h_res);
+    if (sb->done)
+        hipEventDestroy(sb->done);
+    if (sb->stream)
+        hipStreamDestroy(sb->stream);
+    delete sb;
+}
+
+extern "C" int fq_slab_create(int N, int nslots, int dtype, fq_slab **out)
+{
+    if (!out)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (dtype != FQ_C128 && dtype != FQ_C64)
+        return set_err(FQ_ERR_ARG, "dtype must be FQ_C128 or FQ_C64");
+    if (nslots < 1 || (nslots & (nslots - 1)))
+        return set_err(FQ_ERR_ARG, "nslots=%d must be a power of two", nslots);
+    int s = 0;
+    while ((1 << s) < nslots)
+        ++s;
+    if (N < 1 || N + s > 36)
+        return set_err(FQ_ERR_ARG, "N=%d with %d slots out of range", N, nslots);
+    fq_slab *sb = new fq_slab();
+    sb->N = N;
+    sb->K = nslots;
+    sb->s = s;
+    sb->dtype = dtype;
+    sb->elem = dtype == FQ_C128 ? 16 : 8;
+    sb->slot_bytes = ((size_t)1 << N) * sb->elem;
+    sb->bytes = sb->slot_bytes * nslots;
+    sb->slots.resize(nslots);
+    sb->init_amp = pow(1. / sqrt(2.), N);
+    sb->passes = cover_passes(N, tile_max_q(dtype), dtype);
+    if (!budget_take(sb->bytes)) {
+        delete sb;
+        return set_err(FQ_ERR_BUDGET, "slab of %d x 2^%d amplitudes exceeds the memory budget", nslots, N);
+    }
+    // scratch sizes
+    const int Q1 = (int)sb->passes[0].tile.size();
+    Geom g1 = make_geom(N, sb->passes[0].tile, tile_a_log(dtype, Q1));
+    TileArgs a1;
+    geom_to_args(g1, a1);
+    sb->tab_stride = phase_table_entries(a1, g1.a_log);
+    const int Qe = std::min(12, N);
+    const size_t etiles = ((size_t)1 << (N - Qe)) * nslots;
+    bool ok = hipMalloc(&sb->sv, sb->bytes) == hipSuccess;
+    if (!ok) {
+        hipGetLastError();
+        sb->sv = nullptr;
+        budget_give(sb->bytes);
+        slab_free(sb);
+        return set_err(FQ_ERR_OOM, "slab allocation of %zu bytes failed", (size_t)(((size_t)1 << N) * (dtype == FQ_C128 ? 16 : 8) * nslots));
+    }
+    ok = hipMalloc(&sb->d_params, sizeof(SlotParams) * nslots) == hipSuccess &&
+         hipHostMalloc(&sb->h_params, sizeof(SlotParams) * nslots, hipHostMallocDefault) == hipSuccess &&
+         hipMalloc(&sb->d_tabs, sizeof(double2) * sb->tab_stride * nslots) == hipSuccess &&
+         hipMalloc(&sb->d_part, sizeof(double) * 2 * etiles) == hipSuccess &&
+         hipMalloc(&sb->d_res, sizeof(double) * 2 * nslots) == hipSuccess &&
+         hipHostMalloc(&sb->h_res, sizeof(double) * 2 * nslots, hipHostMallocDefault) == hipSuccess &&
+         hipStreamCreateWithFlags(&sb->stream, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&sb->done, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        hipGetLastError();
+        slab_free(sb);
+        return set_err(FQ_ERR_OOM, "slab scratch allocation failed");
+    }
+    memset(sb->h_params, 0, sizeof(SlotParams) * nslots);
+    *out = sb;
+    return FQ_OK;
+}
+
+extern "C" int fq_slab_destroy(fq_slab *sb)
+{
+    if (sb)
+        slab_free(sb);
+    return FQ_OK;
+}
+
+extern "C" int fq_slab_info(const fq_slab *sb, int *N, int *nslots, int *free_slots, int *active)
+{
+    if (!sb)
+        return set_err(FQ_ERR_ARG, "null slab");
+    int fr = 0, ac = 0;
+    for (const SlotHost &h : sb->slots) {
+        fr += h.phase == 0;
+        ac += h.phase == 1;
+    }
+    if (N)
+        *N = sb->N;
+    if (nslots)
+        *nslots = sb->K;
+    if (free_slots)
+        *free_slots = fr;
+    if (active)
+        *active = ac;
+    return FQ_OK;
+}
+
+extern "C" int fq_slab_assign(fq_slab *sb, int slot, fq_graph *g, int P, const double *gammas,
+                              const double *betas)
+{
+    if (!sb || !g)
+        return set_err(FQ_ERR_ARG, "null slab or graph");
+    if (slot < 0 || slot >= sb->K)
+        return set_err(FQ_ERR_ARG, "slot %d out of range", slot);
+    if (g->N != sb->N)
+        return set_err(FQ_ERR_ARG, "graph has N=%d, slab has N=%d", g->N, sb->N);
+    if (P < 1 || !gammas || !betas)
+        return set_err(FQ_ERR_ARG, "bad layer count or angle arrays");
+    for (int p = 0; p < P; ++p)
+        if (!std::isfinite(gammas[p]) || !std::isfinite(betas[p]))
+            return set_err(FQ_ERR_ARG, "angles must be finite");
+    SlotHost &h = sb->slots[slot];
+    if (h.phase != 0)
+        return set_err(FQ_ERR_ARG, "slot %d is not free", slot);
+    h.g = g;
+    h.P = P;
+    h.layer = 0;
+    h.phase = 1;
+    h.gammas.assign(gammas, gammas + P);
+    h.betas.assign(betas, betas + P);
+    return FQ_OK;
+}
+
+extern "C" int fq_slab_poll(fq_slab *sb, int *busy)
+{
+    if (!sb || !busy)
+        return set_err(FQ_ERR_ARG, "null argument");
+    if (!sb->pending) {
+        *busy = 0;
+        return FQ_OK;
+    }
+    hipError_t e = hipEventQuery(sb->done);
+    if (e == hipErrorNotReady) {
+        *busy = 1;
+        return FQ_OK;
+    }
+    if (e != hipSuccess)
+        return set_err(FQ_ERR_DEVICE, "slab step failed: %s", hipGetErrorString(e));
+    sb->pending = false;
+    *busy = 0;
+    return FQ_OK;
+}
+
+// Advance every active slot by one layer: one packed launch per tile pass.
+// Slots that complete their last layer also get <C> reduced; their results
+// are read with fq_slab_collect once fq_slab_poll reports the step done.
+extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
+{
+    if (!sb)
+        return set_err(FQ_ERR_ARG, "null slab");
+    if (sb->pending) {
+        FQ_HIP(hipEventSynchronize(sb->done));
+        sb->pending = false;
+    }
+    const int N = sb->N, K = sb->K;
+    const int Q1 = (int)sb->passes[0].tile.size();
+    const int a1 = tile_a_log(sb->dtype, Q1);
+    Geom gl = make_geom(N, sb->passes[0].tile, a1);
+    const int Qe = std::min(12, N);
+    const int ae = std::max(1, std::min(4, Qe));
+    Geom ge = make_geom(N, low_tile(Qe), ae);
+    int nact = 0, nfin = 0;
+    for (int k = 0; k < K; ++k) {
+        SlotHost &h = sb->slots[k];
+        SlotParams &sp = sb->h_params[k];
+        memset(&sp, 0, sizeof(sp));
+        if (h.phase != 1)
+            continue;
+        const int L = h.layer;
+        double sn, cs;
+        sincos(h.betas[L] / 2, &sn, &cs);
+        sp.rx_c = cs;
+        sp.rx_s = sn;
+        sp.gamma = h.gammas[L];
+        sp.init_amp = sb->init_amp;
+        sp.active = 1;
+        sp.cost_first = L == 0;
+        sp.want_expect = L + 1 == h.P;
+        sp.ptab = sb->d_tabs + sb->tab_stride * (size_t)k;
+        if (int rc = get_plan(h.g, gl, &sp.plan))
+            return rc;
+        if (int rc = get_plan(h.g, ge, &sp.eplan))
+            return rc;
+        for (int j = 0; j < 16; ++j)
+            if (sp.plan.rr[j] != 0)
+                sp.has_rr = 1;
+        ++nact;
+        nfin += sp.want_expect;
+    }
+    if (nfinishing)
+        *nfinishing = nfin;
+    if (nact == 0)
+        return FQ_OK;
+    FQ_HIP(hipMemcpyAsync(sb->d_params, sb->h_params, sizeof(SlotParams) * K, hipMemcpyHostToDevice,
+                          sb->stream));
+    {
+        TileArgs al;
+        geom_to_args(gl, al);
+        FQ_HIP(launch_slab_phase_table(al, a1, sb->d_params, K, sb->d_tabs, sb->tab_stride, sb->stream));
+    }
+    for (size_t i = 0; i < sb->passes.size(); ++i) {
+        const FusedPass &ps = sb->passes[i];
+        const int Q = (int)ps.tile.size();
+        const int a_log = tile_a_log(sb->dtype, Q);
+        Geom gv = make_geom(N + sb->s, ps.tile, a_log);
+        if (geom_runs(gv) > FQ_MAX_RUNS)
+            return set_err(FQ_ERR_ARG, "internal: slab tile needs too many bit runs");
+        TileArgs av;
+        geom_to_args(gv, av);
+        uint32_t gm = 0;
+        for (int q : ps.gates)
+            for (int ti = 0; ti < Q; ++ti)
+                if (ps.tile[ti] == q)
+                    gm |= 1u << ti;
+        if (int rc = set_gates(av, gv, gm))
+            return rc;
+        FQ_HIP(launch_slab_pass(sb->dtype, a_log, i == 0, sb->sv, av, sb->d_params, N - Q, sb->stream));
+    }
+    if (nfin > 0) {
+        Geom gev = make_geom(N + sb->s, low_tile(Qe), ae);
+        TileArgs aev;
+        geom_to_args(gev, aev);
+        FQ_HIP(launch_slab_expect(sb->dtype, ae, sb->sv, aev, sb->d_params, N - Qe, K, sb->d_part, sb->d_res,
+                                  sb->stream));
+        FQ_HIP(hipMemcpyAsync(sb->h_res, sb->d_res, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, sb->stream));
+    }
+    FQ_HIP(hipEventRecord(sb->done, sb->stream));
+    sb->pending = true;
+    for (int k = 0; k < K; ++k) {
+        SlotHost &h = sb->slots[k];
+        if (h.phase != 1)
+            continue;
+        h.layer++;
+        if (h.layer == h.P)
+            h.phase = 2;
+    }
+    return FQ_OK;
+}
+
+// Results of the slots that finished: slot index, <C>, sum |psi|^2.  Waits
+// for the enqueued step, then frees those slots.  n receives the count
+// (capacity: the slab's slot count).
+extern "C" int fq_slab_collect(fq_slab *sb, int *slots, double *expect, double *norm2, int *n)
+{
+    if (!sb || !n)
+        return set_err(FQ_ERR_ARG, "null argument");
+    if (sb->pending) {
+        FQ_HIP(hipEventSynchronize(sb->done));
+        sb->pending = false;
+    }
+    int c = 0;
+    for (int k = 0; k < sb->K; ++k) {
+        SlotHost &h = sb->slots[k];
+        if (h.phase != 2)
+            continue;
+        if (slots)
+            slots[c] = k;
+        if (expect)
+            expect[c] = sb->h_res[2 * k];
+        if (norm2)
+            norm2[c] = sb->h_res[2 * k + 1];
+        ++c;
+        h.phase = 0;
+        h.g = nullptr;
+    }
+    *n = c;
+    return FQ_OK;
+}
+
+// Amplitudes of one slot (identity layout) to host memory.  Waits.
+extern "C" int fq_slab_copy_out(fq_slab *sb, int slot, void *host, size_t bytes)
+{
+    if (!sb || !host)
+        return set_err(FQ_ERR_ARG, "null argument");
+    if (slot < 0 || slot >= sb->K || bytes != sb->slot_bytes)
+        return set_err(FQ_ERR_ARG, "bad slot or size");
+    FQ_HIP(hipMemcpyAsync(host, (char *)sb->sv + sb->slot_bytes * (size_t)slot, bytes, hipMemcpyDeviceToHost,
+                          sb->stream));
+    FQ_HIP(hipStreamSynchronize(sb->stream));
+    sb->pending = false;
     return FQ_OK;
 }

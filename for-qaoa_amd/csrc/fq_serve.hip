@@ -353,9 +353,15 @@ __global__ void __launch_bounds__(256) k_slab_expect(const typename CTS<R>::T *_
 }
 
 // One work-group per slot: sum that slot's tile partials in a fixed order.
+// Only slots reduced in this step are written, so an earlier result stays
+// until it is collected.
 __global__ void __launch_bounds__(256) k_slab_finalize(const double *__restrict__ partials,
-                                                       uint64_t tiles_per_slot, double *__restrict__ out)
+                                                       uint64_t tiles_per_slot,
+                                                       const SlotParams *__restrict__ slots,
+                                                       double *__restrict__ out)
 {
+    if (!slots[blockIdx.x].want_expect)
+        return;
     __shared__ double s_a[256];
     __shared__ double s_b[256];
     const int t = threadIdx.x;
@@ -489,7 +495,7 @@ hipError_t launch_slab_expect(int dtype, int a_log, const void *sv, const TileAr
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_slab_finalize, dim3(nslots), dim3(256), 0, stream, partials,
-                       tiles / (uint64_t)nslots, out);
+                       tiles / (uint64_t)nslots, slots, out);
     return hipGetLastError();
 }
 

@@ -83,6 +83,16 @@ SIGNATURES = {
     "fq_state_gather": (_i, [_vp, _up, _sz, _dp, _i]),
     "fq_sync": (_i, [_vp]),
     "fq_backward": (_i, [_vp, _vp, _i, _dp, _dp, _dp, _dp, _dp]),
+    "fq_set_memory_budget": (_i, [_sz]),
+    "fq_memory_in_use": (_i, [_szp]),
+    "fq_slab_create": (_i, [_i, _i, _i, ctypes.POINTER(_vp)]),
+    "fq_slab_destroy": (_i, [_vp]),
+    "fq_slab_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
+    "fq_slab_assign": (_i, [_vp, _i, _vp, _i, _dp, _dp]),
+    "fq_slab_step": (_i, [_vp, _ip]),
+    "fq_slab_poll": (_i, [_vp, _ip]),
+    "fq_slab_collect": (_i, [_vp, _ip, _dp, _dp, _ip]),
+    "fq_slab_copy_out": (_i, [_vp, _i, _vp, _sz]),
     "fq_timing_enable": (_i, [_vp, _i]),
     "fq_timing_get": (_i, [_vp, _i, _dp, _i64p, _dp]),
     "fq_timing_reset": (_i, [_vp]),

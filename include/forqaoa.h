@@ -163,6 +163,38 @@ int fq_sync(fq_state *st);
 int fq_backward(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
                 double *expect, double *grad_gammas, double *grad_betas);
 
+/* ---- memory budget and packed serving (no reference counterpart) -------- */
+
+/* Cap on the HBM held by states and slabs of this process (0: no cap).
+ * Creating a state, a slab or the adjoint buffer beyond it fails with
+ * FQ_ERR_BUDGET. */
+int fq_set_memory_budget(size_t bytes);
+int fq_memory_in_use(size_t *bytes);
+
+/* A slab packs nslots (a power of two) evaluations of the same N into one
+ * allocation: its slots form one virtual state whose top bits select the
+ * slot, so each tile pass is one launch over every active slot.  Slots are
+ * assigned requests (graph, P, angles) independently; fq_slab_step advances
+ * every active slot by one fused layer (slots at different layers and with
+ * different P run together); a slot finishing its last layer gets <C> reduced
+ * and is collected with fq_slab_collect, after which it is free for the next
+ * request.  Calls on one slab run in order on the slab's own HIP stream. */
+typedef struct fq_slab fq_slab;
+int fq_slab_create(int N, int nslots, int dtype, fq_slab **out);
+int fq_slab_destroy(fq_slab *sb);
+int fq_slab_info(const fq_slab *sb, int *N, int *nslots, int *free_slots, int *active);
+int fq_slab_assign(fq_slab *sb, int slot, fq_graph *g, int P, const double *gammas,
+                   const double *betas);
+/* Enqueue one layer for every active slot (waits for the previous step first);
+ * nfinishing (may be NULL) receives the number of slots finishing with it. */
+int fq_slab_step(fq_slab *sb, int *nfinishing);
+/* busy = 1 while the last step is still running (non-blocking). */
+int fq_slab_poll(fq_slab *sb, int *busy);
+/* Finished slots: indices, <C>, sum |psi|^2; frees them.  Waits. */
+int fq_slab_collect(fq_slab *sb, int *slots, double *expect, double *norm2, int *n);
+/* Amplitudes of one slot (logical = physical order).  Waits. */
+int fq_slab_copy_out(fq_slab *sb, int slot, void *host, size_t bytes);
+
 /* ---- timing (src/rx.cu:139-157 accumulates rx_gate_time_ms the same way) */
 
 /* Enable/disable per-launch HIP event timing on the state's stream. */

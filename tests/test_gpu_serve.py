@@ -1,0 +1,125 @@
+"""Packed slabs and the continuous-batching server.
+
+Each request's <C> in the mix must equal the same request run alone through
+fq_run (fused schedule, its own state): within 1e-5 relative for complex64
+(the stated tolerance of the complex64 path) and 1e-12 for complex128; a
+slot's full state equals the solo state within the same tolerance.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from forqaoa import FQError, Graph, State, _lib
+from forqaoa import graphs as G
+from forqaoa.serve import Server, Slab, latency_stats, make_stream, one_at_a_time
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c128": 1e-12, "c64": 1e-5}
+
+
+def solo(N, edges, g, b, dtype):
+    st = State(N, C=min(N, 13 if dtype == "c128" else 14), dtype=dtype)
+    gr = Graph(N, edges)
+    st.run(gr, g, b, "fused")
+    e, _ = st.expect_cut(gr)
+    amps = st.copy_out()
+    st.destroy()
+    return e, amps
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_slab_slots_match_solo_runs(dtype):
+    N = 15
+    sb = Slab(N, 8, dtype)
+    jobs = []
+    for k in range(5):
+        edges = G.gnp(N, 0.3 + 0.1 * k, 500 + k, "uniform")
+        P = 1 + k
+        g, b = G.angles(P, 600 + k)
+        gr = Graph(N, edges)
+        jobs.append((k, gr, edges, g, b))
+        sb.assign(k, gr, type("R", (), {"gammas": g, "betas": b})())
+    # step until every slot finished; slot k has P = k + 1 layers
+    results = {}
+    amps = {}
+    for step in range(5):
+        sb.step()
+        sb.poll()
+        # copy each slot out right when it finishes (before it is collected)
+        for k, gr, edges, g, b in jobs:
+            if len(g) == step + 1:
+                amps[k] = sb.copy_out(k, dtype)
+        for r in sb.collect():
+            pass
+    for k, gr, edges, g, b in jobs:
+        e_solo, a_solo = solo(N, edges, g, b, dtype)
+        err = np.linalg.norm(amps[k].astype(np.complex128) - a_solo) / np.linalg.norm(a_solo)
+        assert err < TOL[dtype], (k, err)
+    sb.destroy()
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_server_results_equal_solo(dtype):
+    stream = make_stream(60, seed=11, n_range=(12, 18), p_range=(1, 4), rate=None)
+    srv = Server(dtype=dtype, budget_bytes=8e9)
+    done = srv.run(stream, realtime=False)
+    assert len(done) == 60
+    worst = 0.0
+    for r in done:
+        N, edges = stream.graphs[r.graph_key]
+        e_solo, _ = solo(N, edges, r.gammas, r.betas, dtype)
+        worst = max(worst, abs(r.expect - e_solo) / abs(e_solo))
+        assert r.norm2 == pytest.approx(1.0, abs=1e-4 if dtype == "c64" else 1e-11)
+    print(f"server {dtype}: max relative |<C>_mix - <C>_alone| = {worst:.2e}", flush=True)
+    assert worst < TOL[dtype]
+    # requests joined slabs that already had requests in flight
+    assert any(r.joined_running > 0 for r in done)
+    srv.close()
+
+
+def test_server_continuous_admission_realtime():
+    # arrivals while earlier requests run: later requests start before earlier ones finish
+    stream = make_stream(80, seed=12, n_range=(14, 20), p_range=(2, 8), rate=4000.0)
+    srv = Server(dtype="c64", budget_bytes=8e9)
+    done = srv.run(stream, realtime=True)
+    assert len(done) == 80
+    overlaps = sum(1 for a in done for b in done if a.rid < b.rid and b.start < a.done and a.N == b.N)
+    assert overlaps > 0
+    st = latency_stats(done)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
+
+
+def test_budget_limits_admission_and_server_still_finishes():
+    # a budget that holds one slab at a time: slabs are released when idle
+    stream = make_stream(24, seed=13, n_range=(16, 19), p_range=(1, 3), rate=None)
+    srv = Server(dtype="c64", budget_bytes=150e6, slots_for=lambda N: 2)
+    done = srv.run(stream, realtime=False)
+    assert len(done) == 24
+    used = ctypes.c_size_t()
+    _lib.check(_lib.lib().fq_memory_in_use(ctypes.byref(used)))
+    assert used.value <= 150e6
+    srv.close()
+    _lib.check(_lib.lib().fq_set_memory_budget(0))
+
+
+def test_budget_error_code():
+    _lib.check(_lib.lib().fq_set_memory_budget(1 << 20))
+    try:
+        with pytest.raises(FQError) as e:
+            State(20, C=10, dtype="c64")
+        assert e.value.code == _lib.FQ_ERR_BUDGET
+    finally:
+        _lib.check(_lib.lib().fq_set_memory_budget(0))
+
+
+def test_one_at_a_time_baseline_agrees():
+    stream = make_stream(6, seed=14, n_range=(12, 14), p_range=(1, 3))
+    srv = Server(dtype="c128", budget_bytes=4e9)
+    done = {r.rid: r.expect for r in srv.run(stream, realtime=False)}
+    srv.close()
+    one_at_a_time(stream, stream.requests, dtype="c128", schedule="reference")
+    for r in stream.requests:
+        assert r.expect == pytest.approx(done[r.rid], rel=1e-12)
