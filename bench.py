@@ -112,6 +112,60 @@ def cpu_baseline(f=3):
                         f"({t1 - t0:.2f} s), scaled x{1 << f}: {full:.1f} s per full layer"))
 
 
+def serving_bench(n_sat=1000, n_base=150, n_lat=600, seed=20251207):
+    """The serving workload: a stream of 10,000 requests (N uniform in
+    {16..30}, P uniform in {1..10}, G(N, p) with p uniform in [0.2, 0.8],
+    weights in (0, 1], random angles, 80% re-evaluations of resident graphs),
+    complex64, 200 GB HBM budget.  Bounded samples of that stream: the first
+    n_sat requests all queued at t=0 (saturated evals/s), the first n_base of
+    them one at a time with plain launches (the baseline), and n_lat requests
+    arriving as a Poisson stream at 0.8x the saturated rate (p50/p99)."""
+    from forqaoa.serve import Server, bytes_floor, latency_stats, make_stream, one_at_a_time
+
+    full = make_stream(10000, seed=seed)
+    sat = type(full)(graphs=full.graphs, requests=full.requests[:n_sat])
+    for r in sat.requests:
+        r.arrival = 0.0
+    srv = Server(dtype="c64", budget_bytes=200e9)
+    t0 = time.perf_counter()
+    done = srv.run(sat, realtime=False)
+    t_sat = time.perf_counter() - t0
+    srv.close()
+    evals_s = len(done) / t_sat
+    sat_by = {r.rid: r.expect for r in done}
+    base_reqs = [type(r)(r.rid, r.N, r.P, r.graph_key, r.gammas, r.betas) for r in sat.requests[:n_base]]
+    t_base = one_at_a_time(full, base_reqs, dtype="c64", schedule="reference")
+    base_fused = [type(r)(r.rid, r.N, r.P, r.graph_key, r.gammas, r.betas) for r in sat.requests[:n_base]]
+    t_base_f = one_at_a_time(full, base_fused, dtype="c64", schedule="fused")
+    worst = max(abs(r.expect - sat_by[r.rid]) / abs(sat_by[r.rid]) for r in base_reqs)
+    # the same requests through the server, for a like-for-like ratio
+    sub = type(full)(graphs=full.graphs, requests=[type(r)(r.rid, r.N, r.P, r.graph_key, r.gammas, r.betas)
+                                                   for r in sat.requests[:n_base]])
+    srv = Server(dtype="c64", budget_bytes=200e9)
+    t0 = time.perf_counter()
+    srv.run(sub, realtime=False)
+    t_sub = time.perf_counter() - t0
+    srv.close()
+    rate = 0.8 * evals_s
+    lat_stream = make_stream(n_lat, seed=seed + 1, rate=rate)
+    srv = Server(dtype="c64", budget_bytes=200e9)
+    lat_done = srv.run(lat_stream, realtime=True)
+    srv.close()
+    lat = latency_stats(lat_done)
+    floor_b = bytes_floor(full, sat.requests, "c64")
+    return dict(
+        workload="10,000-request stream (N 16..30, P 1..10, G(N,p) p in [0.2,0.8], 80% resident-graph "
+                 "re-evaluations), complex64, 200 GB budget; bounded samples of it",
+        evals_per_s=evals_s, saturated_sample=n_sat, saturated_seconds=t_sat,
+        baseline_sample=n_base, baseline_evals_per_s=n_base / t_base,
+        server_evals_per_s_same_sample=n_base / t_sub,
+        speedup_vs_one_at_a_time=(n_base / t_sub) / (n_base / t_base),
+        one_at_a_time_fused_evals_per_s=n_base / t_base_f,
+        baseline_max_rel_diff=worst,
+        bandwidth_floor_evals_per_s=len(sat.requests) / (floor_b / (HBM_PEAK_GBS * 1e9)),
+        latency_rate_per_s=rate, latency_sample=n_lat, **lat)
+
+
 def traffic_from_profiles():
     if not os.path.exists(PROFILE_TRAFFIC):
         return None
@@ -127,10 +181,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 5 device-to-device copies of a 17.18 GB state (counter calibration)")
     ap.add_argument("--workload", default="n30_gnp05")
     ap.add_argument("--dtype", default="c128")
     ap.add_argument("--schedule", default="fused")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        from forqaoa import State
+
+        a = State(30, C=13, dtype="c128")
+        b = State(30, C=13, dtype="c128")
+        a.init_zero()
+        for _ in range(5):
+            b.clone_from(a)
+        b.sync()
+        print(json.dumps({"calibrate_copy_bytes": a.nbytes, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build simulates one state vector on one GPU; multi-GPU state-vector "
               "transposition is not part of it (run with --gpus 1)", file=sys.stderr)
@@ -151,6 +219,14 @@ def main():
                 extra.append(r)
             except Exception as e:  # report, keep the main line
                 extra.append(dict(workload=name, dtype=dt, schedule=sch, error=str(e)))
+    serving = None
+    if not args.no_serve:
+        try:
+            serving = serving_bench()
+            log(f"serving: {serving['evals_per_s']:.1f} evals/s, x{serving['speedup_vs_one_at_a_time']:.2f} vs "
+                f"one-at-a-time, p50 {serving['p50_ms']:.1f} ms p99 {serving['p99_ms']:.1f} ms")
+        except Exception as e:
+            serving = dict(error=repr(e))
     cpu = None
     if not args.no_cpu:
         try:
@@ -190,6 +266,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": main_res["kernels"],
         "extra": extra,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -1299,6 +1299,21 @@ extern "C" int fq_state_gather(fq_state *st, const uint64_t *idx, size_t n, doub
     return rc;
 }
 
+extern "C" int fq_state_clone(fq_state *dst, const fq_state *src)
+{
+    if (int rc = check_state(dst))
+        return rc;
+    if (int rc = check_state(src))
+        return rc;
+    if (dst->N != src->N || dst->dtype != src->dtype)
+        return set_err(FQ_ERR_ARG, "clone needs equal N and dtype");
+    FQ_HIP(hipStreamSynchronize(src->stream));
+    FQ_HIP(hipMemcpyAsync(dst->sv, src->sv, src->bytes, hipMemcpyDeviceToDevice, dst->stream));
+    for (int p = 0; p < FQ_MAXN; ++p)
+        dst->qmap[p] = src->qmap[p];
+    return FQ_OK;
+}
+
 extern "C" int fq_sync(fq_state *st)
 {
     if (int rc = check_state(st))

@@ -221,6 +221,9 @@ class State:
                                     _dptr(out), int(bool(logical))))
         return out[0::2] + 1j * out[1::2]
 
+    def clone_from(self, other):
+        check(lib().fq_state_clone(self.handle, other.handle))
+
     def sync(self):
         check(lib().fq_sync(self.handle))
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "fq_state_copy_in": (_i, [_vp, _vp, _sz]),
     "fq_state_gather": (_i, [_vp, _up, _sz, _dp, _i]),
     "fq_sync": (_i, [_vp]),
+    "fq_state_clone": (_i, [_vp, _vp]),
     "fq_backward": (_i, [_vp, _vp, _i, _dp, _dp, _dp, _dp, _dp]),
     "fq_set_memory_budget": (_i, [_sz]),
     "fq_memory_in_use": (_i, [_szp]),
