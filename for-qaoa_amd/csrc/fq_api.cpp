@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -812,6 +813,19 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
 
 static int carriers_for(int dtype) { return dtype == FQ_C128 ? 3 : 4; }
 
+// Tile width of the fused and strided passes: the LDS limit by default;
+// FQ_TILE_Q (environment) may lower it (smaller tiles: two work-groups per CU).
+static int pass_tile_q(int dtype)
+{
+    static int env_q = -1;
+    if (env_q < 0) {
+        const char *e = getenv("FQ_TILE_Q");
+        env_q = e ? atoi(e) : 0;
+    }
+    const int mx = tile_max_q(dtype);
+    return env_q > 0 && env_q < mx ? env_q : mx;
+}
+
 // Tile passes covering every qubit once: a contiguous low tile of Qmax
 // qubits, then the rest spread evenly over strided tiles that carry the
 // lowest qubits (at least 128 contiguous bytes per run).
@@ -860,7 +874,7 @@ static int rx_targets(fq_state *st, std::vector<int> targets, double beta)
         return FQ_OK;
     if (targets.back() < Ceff)
         return tile_pass(st, low_tile(Ceff), targets, beta, nullptr, 0, false, false, FQ_K_RX);
-    const int Qmax = std::min(tile_max_q(st->dtype), N);
+    const int Qmax = std::min(pass_tile_q(st->dtype), N);
     const int per = std::max(1, Qmax - std::min(carriers_for(st->dtype), Qmax - 1));
     size_t pos = 0;
     while (pos < targets.size()) {
@@ -964,7 +978,7 @@ extern "C" int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_sw
 // (contiguous tile, cost phase fused), the others are strided.
 static std::vector<FusedPass> fused_passes(int N, int dtype)
 {
-    return cover_passes(N, tile_max_q(dtype), dtype);
+    return cover_passes(N, pass_tile_q(dtype), dtype);
 }
 
 extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,

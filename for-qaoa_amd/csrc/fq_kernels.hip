@@ -750,8 +750,12 @@ static hipError_t launch_tile_r(int a_log, bool cost, bool first, bool gates, vo
     // cost passes; anything else takes the generic instance
     if (sizeof(R) == 8 && a_log == 3 && a.Q == 13 && gates)
         return launch_tile_flags<R, 3, 13>(cost, first, gates, sv, a, ptab, stream);
+    if (sizeof(R) == 8 && a_log == 3 && a.Q == 12 && gates)
+        return launch_tile_flags<R, 3, 12>(cost, first, gates, sv, a, ptab, stream);
     if (sizeof(R) == 4 && a_log == 4 && a.Q == 14 && gates)
         return launch_tile_flags<R, 4, 14>(cost, first, gates, sv, a, ptab, stream);
+    if (sizeof(R) == 4 && a_log == 4 && a.Q == 13 && gates)
+        return launch_tile_flags<R, 4, 13>(cost, first, gates, sv, a, ptab, stream);
     if (a_log == 4 && a.Q == 12 && !gates)
         return launch_tile_flags<R, 4, 12>(cost, first, gates, sv, a, ptab, stream);
     switch (a_log) {
