@@ -13,7 +13,9 @@ state resident in HBM.  value = 5 * steps / seconds.
 
 Also reported (extra): the same evaluation on the 3-regular graph, in
 complex64, with the reference schedule (cost, RX chunks, transpositions), and
-N=34 P=10 complex64 on G(34, 0.3) with integer weights.
+N=34 P=10 complex64 on G(34, 0.3) with integer weights, and the half-state
+schedule (layers on the z_{N-1} = 0 half, psi(b) = psi(~b); the full state is
+written after the last layer) for N=30 and N=34.
 
 cpu_baseline: the oracle (the C restatement of the reference's layer loop,
 complex128, OpenMP) times one layer of the same N=30 instance on a bounded
@@ -146,23 +148,34 @@ def serving_bench(n_sat=1000, n_base=150, n_lat=600, seed=20251207):
     srv.run(sub, realtime=False)
     t_sub = time.perf_counter() - t0
     srv.close()
+    # the same sample through full-state slabs
+    sub_f = type(full)(graphs=full.graphs, requests=[type(r)(r.rid, r.N, r.P, r.graph_key, r.gammas, r.betas)
+                                                     for r in sat.requests[:n_base]])
+    srv = Server(dtype="c64", budget_bytes=200e9, schedule="fused")
+    t0 = time.perf_counter()
+    srv.run(sub_f, realtime=False)
+    t_sub_f = time.perf_counter() - t0
+    srv.close()
     rate = 0.8 * evals_s
     lat_stream = make_stream(n_lat, seed=seed + 1, rate=rate)
     srv = Server(dtype="c64", budget_bytes=200e9)
     lat_done = srv.run(lat_stream, realtime=True)
     srv.close()
     lat = latency_stats(lat_done)
-    floor_b = bytes_floor(full, sat.requests, "c64")
+    floor_b = bytes_floor(full, sat.requests, "c64", "z2")
+    floor_f = bytes_floor(full, sat.requests, "c64", "fused")
     return dict(
         workload="10,000-request stream (N 16..30, P 1..10, G(N,p) p in [0.2,0.8], 80% resident-graph "
-                 "re-evaluations), complex64, 200 GB budget; bounded samples of it",
+                 "re-evaluations), complex64, 200 GB budget, half-state slabs; bounded samples of it",
         evals_per_s=evals_s, saturated_sample=n_sat, saturated_seconds=t_sat,
         baseline_sample=n_base, baseline_evals_per_s=n_base / t_base,
         server_evals_per_s_same_sample=n_base / t_sub,
         speedup_vs_one_at_a_time=(n_base / t_sub) / (n_base / t_base),
         one_at_a_time_fused_evals_per_s=n_base / t_base_f,
+        server_full_state_slabs_evals_per_s_same_sample=n_base / t_sub_f,
         baseline_max_rel_diff=worst,
         bandwidth_floor_evals_per_s=len(sat.requests) / (floor_b / (HBM_PEAK_GBS * 1e9)),
+        bandwidth_floor_full_state_evals_per_s=len(sat.requests) / (floor_f / (HBM_PEAK_GBS * 1e9)),
         latency_rate_per_s=rate, latency_sample=n_lat, **lat)
 
 
@@ -211,7 +224,10 @@ def main():
         for name, dt, sch, steps, warm in [("n30_reg3", "c128", "fused", 3, 1),
                                             ("n30_gnp05", "c64", "fused", 3, 1),
                                             ("n30_gnp05", "c128", "reference", 2, 1),
-                                            ("n34_gnp03_int", "c64", "fused", 1, 1)]:
+                                            ("n34_gnp03_int", "c64", "fused", 1, 1),
+                                            ("n30_gnp05", "c128", "z2", 3, 1),
+                                            ("n30_gnp05", "c64", "z2", 3, 1),
+                                            ("n34_gnp03_int", "c64", "z2", 1, 1)]:
             try:
                 r = measure(name, dt, sch, steps, warm)
                 log(f"extra {name} {dt} {sch}: {r['layers_per_s']:.2f} layers/s")

@@ -238,9 +238,22 @@ static int geom_runs(const Geom &g)
     return std::max(make_runs(g.hpos, g.nh, r), make_runs(g.tpos, g.nthr, r));
 }
 
+// Tile order of the persistent passes (tile_at); FQ_TILE_ORDER (environment)
+// selects another for experiments.
+static uint32_t tile_order()
+{
+    static int order = -1;
+    if (order < 0) {
+        const char *e = getenv("FQ_TILE_ORDER");
+        order = e ? std::max(0, std::min(3, atoi(e))) : 0;
+    }
+    return (uint32_t)order;
+}
+
 static void geom_to_args(const Geom &g, TileArgs &a)
 {
     memset(&a, 0, sizeof(a));
+    a.order = tile_order();
     a.N = g.N;
     a.Q = g.Q;
     a.nthr = g.nthr;
@@ -770,9 +783,15 @@ static int cost_pass(fq_state *st, fq_graph *g, double gamma, bool first)
     });
 }
 
+// Half-state (Z2) modes of a tile pass: tiles with z_{N-1} = 0 only, or the
+// mirror pass whose top register bit is qubit N-1 (Z2_EXPAND: and that
+// writes the full state's upper half, after the last layer).
+enum { Z2_OFF = 0, Z2_LOWER = 1, Z2_MIRROR = 2, Z2_EXPAND = 3 };
+
 // One tile pass with RX on gate positions (and optionally the cost phase).
 static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vector<int> &gates,
-                     double beta, fq_graph *g, double gamma, bool cost, bool first, int kind)
+                     double beta, fq_graph *g, double gamma, bool cost, bool first, int kind,
+                     int z2 = Z2_OFF)
 {
     const int Q = (int)tile.size();
     const int a_log = tile_a_log(st->dtype, Q);
@@ -793,6 +812,18 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     }
     if (int rc = set_gates(a, geom, gm))
         return rc;
+    if (z2 != Z2_OFF) {
+        // the lower half is the tiles whose top index bit (qubit N-1) is 0;
+        // a mirror pass pairs tile tau with ~tau instead
+        const bool ok = z2 == Z2_LOWER ? geom.nh >= 1 && geom.hpos[geom.nh - 1] == st->N - 1
+                                       : geom.nh >= 1 && tile.back() == st->N - 1 && !cost;
+        if (!ok)
+            return set_err(FQ_ERR_ARG, "internal: tile does not fit the half-state pass");
+        a.ntiles_run = 1ull << (geom.nh - 1);
+        if (z2 >= Z2_MIRROR)
+            for (int i = 0; i + 1 < Q; ++i)
+                a.mirror_pm += 1ull << tile[i];
+    }
     double s, c;
     sincos(beta / 2, &s, &c);
     a.rx_c = c;
@@ -804,8 +835,12 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
         if (int rc = prepare_phases(st, a, a_log))
             return rc;
     }
-    const double bytes = (first ? 1.0 : 2.0) * (double)st->bytes;
+    // a half-state pass reads and writes half the state; the expanding one
+    // writes the whole state
+    const double bytes = (double)st->bytes * (z2 == Z2_OFF ? (first ? 1.0 : 2.0) : z2 == Z2_EXPAND ? 1.5 : 1.0);
     return timed(st, kind, bytes, [&] {
+        if (z2 >= Z2_MIRROR)
+            return launch_tile_mirror(st->dtype, a_log, z2 == Z2_EXPAND, st->sv, a, st->stream);
         return launch_tile(st->dtype, a_log, cost, first, true, st->sv, a, cost ? st->d_phase : nullptr,
                            st->stream);
     });
@@ -1016,6 +1051,24 @@ extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, co
                     if (int rc = fq_graph_relabel(g, 0, swap_in, sizes[i + 1]))
                         return rc;
                 }
+            }
+        }
+        return FQ_OK;
+    }
+    if (schedule == FQ_SCHED_Z2) {
+        // Cut(b) = Cut(~b), RX^{(x)N} commutes with X^{(x)N} and |+> is
+        // X^{(x)N}-invariant, so psi(b) = psi(~b): the layers run on the
+        // half z_{N-1} = 0 and the other half is written once at the end.
+        if (N < 3)
+            return set_err(FQ_ERR_ARG, "the half-state schedule needs N >= 3");
+        const std::vector<FusedPass> ps = cover_passes(N, std::min(pass_tile_q(st->dtype), N - 1), st->dtype);
+        for (int p = 0; p < P; ++p) {
+            for (size_t i = 0; i < ps.size(); ++i) {
+                const bool cost = i == 0;
+                const int mode = i + 1 < ps.size() ? Z2_LOWER : p + 1 < P ? Z2_MIRROR : Z2_EXPAND;
+                if (int rc = tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, gammas[p], cost,
+                                       cost && p == 0, FQ_K_FUSED, mode))
+                    return rc;
             }
         }
         return FQ_OK;
@@ -1386,6 +1439,7 @@ struct SlotHost {
 
 struct fq_slab {
     int N = 0, K = 0, s = 0, dtype = 0;
+    bool z2 = false;  // slots hold the half state z_{N-1} = 0
     size_t elem = 8, slot_bytes = 0, bytes = 0;
     void *sv = nullptr;
     hipStream_t stream = nullptr;
@@ -1421,13 +1475,38 @@ static void slab_free(fq_slab *sb)
     delete sb;
 }
 
+// Reduction tile width of a slab's <C> (a half-state slab reduces its N-1 stored qubits).
+static int slab_qe(const fq_slab *sb) { return std::min(12, sb->z2 ? sb->N - 1 : sb->N); }
+
+// Phase-table arguments of a slab's cost pass: the N-qubit geometry of pass 1
+// (a half-state slab uses its lower half of the tiles).
+static TileArgs slab_cost_args(const fq_slab *sb, Geom *gout)
+{
+    const int Q1 = (int)sb->passes[0].tile.size();
+    Geom g1 = make_geom(sb->N, sb->passes[0].tile, tile_a_log(sb->dtype, Q1));
+    TileArgs a1;
+    geom_to_args(g1, a1);
+    if (sb->z2)
+        a1.ntiles_run = 1ull << (g1.nh - 1);
+    if (gout)
+        *gout = g1;
+    return a1;
+}
+
 extern "C" int fq_slab_create(int N, int nslots, int dtype, fq_slab **out)
+{
+    return fq_slab_create_sched(N, nslots, dtype, FQ_SCHED_FUSED, out);
+}
+
+extern "C" int fq_slab_create_sched(int N, int nslots, int dtype, int schedule, fq_slab **out)
 {
     if (!out)
         return set_err(FQ_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (dtype != FQ_C128 && dtype != FQ_C64)
         return set_err(FQ_ERR_ARG, "dtype must be FQ_C128 or FQ_C64");
+    if (schedule != FQ_SCHED_FUSED && schedule != FQ_SCHED_Z2)
+        return set_err(FQ_ERR_ARG, "slab schedule must be FQ_SCHED_FUSED or FQ_SCHED_Z2");
     if (nslots < 1 || (nslots & (nslots - 1)))
         return set_err(FQ_ERR_ARG, "nslots=%d must be a power of two", nslots);
     int s = 0;
@@ -1435,29 +1514,31 @@ extern "C" int fq_slab_create(int N, int nslots, int dtype, fq_slab **out)
         ++s;
     if (N < 1 || N + s > 36)
         return set_err(FQ_ERR_ARG, "N=%d with %d slots out of range", N, nslots);
+    const bool z2 = schedule == FQ_SCHED_Z2;
+    if (z2 && N < 3)
+        return set_err(FQ_ERR_ARG, "the half-state schedule needs N >= 3");
     fq_slab *sb = new fq_slab();
     sb->N = N;
     sb->K = nslots;
     sb->s = s;
     sb->dtype = dtype;
+    sb->z2 = z2;
     sb->elem = dtype == FQ_C128 ? 16 : 8;
-    sb->slot_bytes = ((size_t)1 << N) * sb->elem;
+    sb->slot_bytes = ((size_t)1 << (z2 ? N - 1 : N)) * sb->elem;
     sb->bytes = sb->slot_bytes * nslots;
     sb->slots.resize(nslots);
     sb->init_amp = pow(1. / sqrt(2.), N);
-    sb->passes = cover_passes(N, tile_max_q(dtype), dtype);
+    sb->passes = cover_passes(N, std::min(tile_max_q(dtype), z2 ? N - 1 : N), dtype);
     if (!budget_take(sb->bytes)) {
         delete sb;
         return set_err(FQ_ERR_BUDGET, "slab of %d x 2^%d amplitudes exceeds the memory budget", nslots, N);
     }
     // scratch sizes
-    const int Q1 = (int)sb->passes[0].tile.size();
-    Geom g1 = make_geom(N, sb->passes[0].tile, tile_a_log(dtype, Q1));
-    TileArgs a1;
-    geom_to_args(g1, a1);
+    Geom g1;
+    TileArgs a1 = slab_cost_args(sb, &g1);
     sb->tab_stride = phase_table_entries(a1, g1.a_log);
-    const int Qe = std::min(12, N);
-    const size_t etiles = ((size_t)1 << (N - Qe)) * nslots;
+    const int Qe = slab_qe(sb);
+    const size_t etiles = ((size_t)1 << ((z2 ? N - 1 : N) - Qe)) * nslots;
     bool ok = hipMalloc(&sb->sv, sb->bytes) == hipSuccess;
     if (!ok) {
         hipGetLastError();
@@ -1569,10 +1650,12 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         sb->pending = false;
     }
     const int N = sb->N, K = sb->K;
-    const int Q1 = (int)sb->passes[0].tile.size();
-    const int a1 = tile_a_log(sb->dtype, Q1);
-    Geom gl = make_geom(N, sb->passes[0].tile, a1);
-    const int Qe = std::min(12, N);
+    // stored qubits per slot: N, or N-1 for a half-state slab
+    const int Ns = sb->z2 ? N - 1 : N;
+    Geom gl;
+    TileArgs al = slab_cost_args(sb, &gl);
+    const int a1 = gl.a_log;
+    const int Qe = slab_qe(sb);
     const int ae = std::max(1, std::min(4, Qe));
     Geom ge = make_geom(N, low_tile(Qe), ae);
     int nact = 0, nfin = 0;
@@ -1609,16 +1692,18 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         return FQ_OK;
     FQ_HIP(hipMemcpyAsync(sb->d_params, sb->h_params, sizeof(SlotParams) * K, hipMemcpyHostToDevice,
                           sb->stream));
-    {
-        TileArgs al;
-        geom_to_args(gl, al);
-        FQ_HIP(launch_slab_phase_table(al, a1, sb->d_params, K, sb->d_tabs, sb->tab_stride, sb->stream));
-    }
+    FQ_HIP(launch_slab_phase_table(al, a1, sb->d_params, K, sb->d_tabs, sb->tab_stride, sb->stream));
     for (size_t i = 0; i < sb->passes.size(); ++i) {
         const FusedPass &ps = sb->passes[i];
         const int Q = (int)ps.tile.size();
         const int a_log = tile_a_log(sb->dtype, Q);
-        Geom gv = make_geom(N + sb->s, ps.tile, a_log);
+        // virtual state: the Ns stored qubits, then the slot bits; the mirror
+        // pass of a half-state slab keeps qubit N-1 as a position above them
+        const bool mirror = sb->z2 && i + 1 == sb->passes.size();
+        std::vector<int> tile = ps.tile;
+        if (mirror)
+            tile.back() = Ns + sb->s;
+        Geom gv = make_geom(Ns + sb->s + (mirror ? 1 : 0), tile, a_log);
         if (geom_runs(gv) > FQ_MAX_RUNS)
             return set_err(FQ_ERR_ARG, "internal: slab tile needs too many bit runs");
         TileArgs av;
@@ -1630,13 +1715,18 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
                     gm |= 1u << ti;
         if (int rc = set_gates(av, gv, gm))
             return rc;
-        FQ_HIP(launch_slab_pass(sb->dtype, a_log, i == 0, sb->sv, av, sb->d_params, N - Q, sb->stream));
+        if (mirror)
+            for (int ti = 0; ti + 1 < Q; ++ti)
+                av.mirror_pm += 1ull << tile[ti];
+        const int slot_shift = mirror ? N - Q : Ns - Q;
+        FQ_HIP(launch_slab_pass(sb->dtype, a_log, i == 0, mirror, sb->sv, av, sb->d_params, slot_shift,
+                                sb->stream));
     }
     if (nfin > 0) {
-        Geom gev = make_geom(N + sb->s, low_tile(Qe), ae);
+        Geom gev = make_geom(Ns + sb->s, low_tile(Qe), ae);
         TileArgs aev;
         geom_to_args(gev, aev);
-        FQ_HIP(launch_slab_expect(sb->dtype, ae, sb->sv, aev, sb->d_params, N - Qe, K, sb->d_part, sb->d_res,
+        FQ_HIP(launch_slab_expect(sb->dtype, ae, sb->sv, aev, sb->d_params, Ns - Qe, K, sb->d_part, sb->d_res,
                                   sb->stream));
         FQ_HIP(hipMemcpyAsync(sb->h_res, sb->d_res, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, sb->stream));
     }
@@ -1671,10 +1761,12 @@ extern "C" int fq_slab_collect(fq_slab *sb, int *slots, double *expect, double *
             continue;
         if (slots)
             slots[c] = k;
+        // a half-state slab holds one of the two equal halves
+        const double f = sb->z2 ? 2.0 : 1.0;
         if (expect)
-            expect[c] = sb->h_res[2 * k];
+            expect[c] = f * sb->h_res[2 * k];
         if (norm2)
-            norm2[c] = sb->h_res[2 * k + 1];
+            norm2[c] = f * sb->h_res[2 * k + 1];
         ++c;
         h.phase = 0;
         h.g = nullptr;
@@ -1688,11 +1780,19 @@ extern "C" int fq_slab_copy_out(fq_slab *sb, int slot, void *host, size_t bytes)
 {
     if (!sb || !host)
         return set_err(FQ_ERR_ARG, "null argument");
-    if (slot < 0 || slot >= sb->K || bytes != sb->slot_bytes)
-        return set_err(FQ_ERR_ARG, "bad slot or size");
-    FQ_HIP(hipMemcpyAsync(host, (char *)sb->sv + sb->slot_bytes * (size_t)slot, bytes, hipMemcpyDeviceToHost,
-                          sb->stream));
+    const size_t full = ((size_t)1 << sb->N) * sb->elem;
+    if (slot < 0 || slot >= sb->K || bytes != full)
+        return set_err(FQ_ERR_ARG, "bad slot or size (%zu bytes expected)", full);
+    FQ_HIP(hipMemcpyAsync(host, (char *)sb->sv + sb->slot_bytes * (size_t)slot, sb->slot_bytes,
+                          hipMemcpyDeviceToHost, sb->stream));
     FQ_HIP(hipStreamSynchronize(sb->stream));
     sb->pending = false;
+    if (sb->z2) {
+        // psi(b) = psi(~b): the upper half is the stored half reversed
+        const size_t half = (size_t)1 << (sb->N - 1), e = sb->elem;
+        char *h = (char *)host;
+        for (size_t i = 0; i < half; ++i)
+            memcpy(h + (half + i) * e, h + (half - 1 - i) * e, e);
+    }
     return FQ_OK;
 }

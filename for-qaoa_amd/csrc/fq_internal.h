@@ -57,6 +57,9 @@ struct TileArgs {
     BitRuns hruns;           // tile index -> high positions
     BitRuns truns;           // thread index -> positions of the thread bits
     uint8_t rpos[4];         // positions of the register bits
+    uint64_t ntiles_run;     // tiles processed (0: all 2^nh)
+    uint64_t mirror_pm;      // mirror pass: sum of 2^p over the real tile positions
+    uint32_t order;          // tile order of the persistent loop (tile_at)
     uint32_t gate_mask;      // over tile bits
     int ngroups;
     uint16_t grp_mask[FQ_MAX_GROUPS];  // tile-bit mask of each LDS phase (A_LOG bits)
@@ -66,6 +69,12 @@ struct TileArgs {
     int has_rr;              // any register-register edge in the plan
     PlanDev plan;
 };
+
+// Tiles a launch processes (a Z2-reduced state runs the lower half only).
+__host__ __device__ inline uint64_t run_tiles(const TileArgs &a)
+{
+    return a.ntiles_run ? a.ntiles_run : (1ull << a.nh);
+}
 
 // Per-slot parameters of a packed (slab) step, in device memory.
 struct SlotParams {
@@ -93,6 +102,12 @@ struct SwapArgs {
 // launch status.
 hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, void *sv,
                        const TileArgs &a, const double2 *phase_tab, hipStream_t stream);
+// Mirror pass of a Z2-reduced state: the top register bit of the virtual tile
+// selects the complementary tile (complemented element offsets) and RX on it
+// is the butterfly of the global spin flip.  expand: also write the upper
+// half of the full state.
+hipError_t launch_tile_mirror(int dtype, int a_log, bool expand, void *sv, const TileArgs &a,
+                              hipStream_t stream);
 // Phase factors of a cost-phase launch into tab (size phase_table_entries).
 hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, hipStream_t stream);
 size_t phase_table_entries(const TileArgs &a, int a_log);
@@ -115,7 +130,7 @@ size_t pair_lds_bytes(int dtype, int Q);
 // packed slabs (fq_serve.hip)
 hipError_t launch_slab_phase_table(const TileArgs &local, int a_log, const SlotParams *slots, int nslots,
                                    double2 *tabs, size_t stride, hipStream_t stream);
-hipError_t launch_slab_pass(int dtype, int a_log, bool cost, void *sv, const TileArgs &a,
+hipError_t launch_slab_pass(int dtype, int a_log, bool cost, bool mirror, void *sv, const TileArgs &a,
                             const SlotParams *slots, int slot_shift, hipStream_t stream);
 hipError_t launch_slab_expect(int dtype, int a_log, const void *sv, const TileArgs &a, const SlotParams *slots,
                               int slot_shift, int nslots, double *partials, double *out, hipStream_t stream);

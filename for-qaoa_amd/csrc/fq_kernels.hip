@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(256) k_phase_table(const TileArgs a, int a_log
     const PlanDev &p = a.plan;
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t q1 = (uint64_t)a.Q + 1;
-    const uint64_t ntile_e = (1ull << a.nh) * q1;
+    const uint64_t ntile_e = run_tiles(a) * q1;
     const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
     double angle;
     if (gid < ntile_e) {
@@ -225,12 +225,31 @@ __device__ __forceinline__ int deposit(int x, unsigned mask, int q)
     return e;
 }
 
+// k-th tile of this work-group in a persistent pass of G = gridDim.x groups:
+//   order 0: k*G + b (the groups sweep the tiles together),
+//   order 1: the groups of one XCD (dispatched round-robin, b mod 8) take
+//            consecutive tiles of each sweep,
+//   order 2: one contiguous range of tiles per group,
+//   order 3: as 0 with one work-group per tile (no persistence).
+// Returns ntiles or more once the group has no tile left.
+__device__ __forceinline__ uint64_t tile_at(const TileArgs &a, uint64_t ntiles, uint64_t k)
+{
+    const uint64_t G = gridDim.x, b = blockIdx.x;
+    if (a.order == 1)
+        return k * G + (b & 7) * (G >> 3) + (b >> 3);
+    if (a.order == 2) {
+        const uint64_t per = (ntiles + G - 1) / G;
+        return k < per ? b * per + k : ntiles;
+    }
+    return k * G + b;
+}
+
 // Persistent tile pass.  Each work-group walks tiles blockIdx.x,
 // blockIdx.x + gridDim.x, ...; while a tile goes through its cost phase and
 // LDS phases, the next tile's amplitudes are already loading into registers.
 // QT > 0 fixes the tile width at compile time (the fused passes' full-width
 // tiles); QT == 0 reads it from the arguments.
-template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES>
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0>
 __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a,
                                                             const double2 *__restrict__ ptab)
 {
@@ -242,7 +261,13 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     const int Q = QT > 0 ? QT : a.Q;
     const int nthr = QT > 0 ? QT - A_LOG : a.nthr;
     const int t = threadIdx.x;
-    const uint64_t ntiles = 1ull << a.nh;
+    // MIRROR: iterate tile pairs (tau, ~tau); the upper half of the register
+    // index addresses the complementary tile at complemented offsets.
+    // MIRROR == 2 also writes the upper half of the full state
+    // (psi(x | 2^(N-1)) = psi(~x)) from the same registers.
+    const uint64_t ntiles = run_tiles(a);
+    const uint64_t hmask = (1ull << a.nh) - 1ull;
+    constexpr int AH = MIRROR ? (1 << A_LOG) / 2 : (1 << A_LOG);
 
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     uint64_t roff[A];
@@ -250,7 +275,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     for (int j = 0; j < A; ++j) {
         uint64_t o = 0;
 #pragma unroll
-        for (int r = 0; r < A_LOG; ++r)
+        for (int r = 0; r < (MIRROR ? A_LOG - 1 : A_LOG); ++r)
             if ((j >> r) & 1)
                 o |= 1ull << a.rpos[r];
         roff[j] = o;
@@ -270,16 +295,23 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     const double2 *thr_tab = COST ? ptab + (ntiles * q1) : nullptr;
     const double2 *rr_tab = COST ? thr_tab + ((size_t)(1 + A_LOG) << nthr) : nullptr;
 
-    uint64_t tile = blockIdx.x;
+    uint64_t kt = 0;
+    uint64_t tile = tile_at(a, ntiles, 0);
     T v[A];
     if (!FIRST && tile < ntiles) {
         const T *p0 = sv + (tile_base(a, tile) + toff);
 #pragma unroll
-        for (int j = 0; j < A; ++j)
+        for (int j = 0; j < AH; ++j)
             v[j] = p0[roff[j]];
+        if (MIRROR) {
+            const T *pm = sv + (tile_base(a, ~tile & hmask) + a.mirror_pm - toff);
+#pragma unroll
+            for (int j = AH; j < A; ++j)
+                v[j] = pm[-(int64_t)roff[j - AH]];
+        }
     }
-    for (; tile < ntiles; tile += gridDim.x) {
-        const uint64_t next = tile + gridDim.x;
+    while (tile < ntiles) {
+        const uint64_t next = tile_at(a, ntiles, ++kt);
 
         // Cost phase first (few registers live), then the prefetch.
         // exp(i*gamma*s/2), s = 2*Cut - W, of register j is the product of the
@@ -339,8 +371,14 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         if (!FIRST && next < ntiles) {
             const T *p1 = sv + (tile_base(a, next) + toff);
 #pragma unroll
-            for (int j = 0; j < A; ++j)
+            for (int j = 0; j < AH; ++j)
                 w[j] = p1[roff[j]];
+            if (MIRROR) {
+                const T *pm = sv + (tile_base(a, ~next & hmask) + a.mirror_pm - toff);
+#pragma unroll
+                for (int j = AH; j < A; ++j)
+                    w[j] = pm[-(int64_t)roff[j - AH]];
+            }
         }
 
         if (GATES) {
@@ -395,14 +433,30 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         {
             T *const p0 = sv + (tile_base(a, tile) + toff);
 #pragma unroll
-            for (int j = 0; j < A; ++j)
+            for (int j = 0; j < AH; ++j)
                 p0[roff[j]] = v[j];
+            if (MIRROR) {
+                T *const pm = sv + (tile_base(a, ~tile & hmask) + a.mirror_pm - toff);
+#pragma unroll
+                for (int j = AH; j < A; ++j)
+                    pm[-(int64_t)roff[j - AH]] = v[j];
+                if (MIRROR == 2) {
+                    const uint64_t half = 1ull << (a.N - 1);
+#pragma unroll
+                    for (int j = 0; j < AH; ++j)
+                        p0[half + roff[j]] = v[j + AH];
+#pragma unroll
+                    for (int j = AH; j < A; ++j)
+                        pm[half - roff[j - AH]] = v[j - AH];
+                }
+            }
         }
         if (!FIRST && next < ntiles) {
 #pragma unroll
             for (int j = 0; j < A; ++j)
                 v[j] = w[j];
         }
+        tile = next;
     }
 }
 
@@ -704,7 +758,7 @@ static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab
 {
     typedef typename CT<R>::T T;
     const unsigned threads = 1u << a.nthr;
-    const uint64_t tiles = 1ull << a.nh;
+    const uint64_t tiles = run_tiles(a);
     const size_t lds = GATES && a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
     const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES>);
     if (lds > 65536) {
@@ -723,47 +777,141 @@ static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab
         per_cu < 1)
         per_cu = 1;
     const uint64_t resident = (uint64_t)per_cu * device_cus();
-    const uint64_t grid = tiles < resident ? tiles : resident;
+    // order 3: one tile per work-group (the dispatcher keeps a sliding window)
+    const uint64_t grid = tiles < resident || a.order == 3 ? tiles : resident;
+    TileArgs la = a;
+    if (la.order == 1 && grid % 8 != 0)
+        la.order = 0;
     hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES>), dim3((unsigned)grid), dim3(threads),
-                       lds, stream, reinterpret_cast<T *>(sv), a, ptab);
+                       lds, stream, reinterpret_cast<T *>(sv), la, ptab);
+    return hipGetLastError();
+}
+
+template <typename R, int A_LOG, int QT, int MIRROR>
+static hipError_t launch_mirror_m(void *sv, const TileArgs &a, hipStream_t stream)
+{
+    typedef typename CT<R>::T T;
+    const unsigned threads = 1u << a.nthr;
+    const uint64_t tiles = a.ntiles_run;
+    const size_t lds = a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
+    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, false, false, true, MIRROR>);
+    if (lds > 65536) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024 - 1024);
+            if (e != hipSuccess)
+                return e;
+            attr_set = true;
+        }
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint64_t resident = (uint64_t)per_cu * device_cus();
+    const uint64_t grid = tiles < resident || a.order == 3 ? tiles : resident;
+    TileArgs la = a;
+    if (la.order == 1 && grid % 8 != 0)
+        la.order = 0;
+    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, false, false, true, MIRROR>), dim3((unsigned)grid), dim3(threads),
+                       lds, stream, reinterpret_cast<T *>(sv), la, nullptr);
     return hipGetLastError();
 }
 
 template <typename R, int A_LOG, int QT>
-static hipError_t launch_tile_flags(bool cost, bool first, bool gates, void *sv, const TileArgs &a,
-                                    const double2 *ptab, hipStream_t stream)
+static hipError_t launch_mirror_t(bool expand, void *sv, const TileArgs &a, hipStream_t stream)
+{
+    return expand ? launch_mirror_m<R, A_LOG, QT, 2>(sv, a, stream) : launch_mirror_m<R, A_LOG, QT, 1>(sv, a, stream);
+}
+
+hipError_t launch_tile_mirror(int dtype, int a_log, bool expand, void *sv, const TileArgs &a, hipStream_t stream)
+{
+    if (a.ntiles_run == 0)
+        return hipErrorInvalidValue;
+    if (dtype == 0) {
+        if (a_log == 3 && a.Q == 13)
+            return launch_mirror_t<double, 3, 13>(expand, sv, a, stream);
+        switch (a_log) {
+            case 1: return launch_mirror_t<double, 1, 0>(expand, sv, a, stream);
+            case 2: return launch_mirror_t<double, 2, 0>(expand, sv, a, stream);
+            case 3: return launch_mirror_t<double, 3, 0>(expand, sv, a, stream);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    if (a_log == 4 && a.Q == 14)
+        return launch_mirror_t<float, 4, 14>(expand, sv, a, stream);
+    switch (a_log) {
+        case 1: return launch_mirror_t<float, 1, 0>(expand, sv, a, stream);
+        case 2: return launch_mirror_t<float, 2, 0>(expand, sv, a, stream);
+        case 3: return launch_mirror_t<float, 3, 0>(expand, sv, a, stream);
+        case 4: return launch_mirror_t<float, 4, 0>(expand, sv, a, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Cost phase without gates (the reference's separate cost kernel): 256
+// threads x up to 16 registers, no LDS.
+template <typename R, int A_LOG, int QT>
+static hipError_t launch_cost_only(bool first, void *sv, const TileArgs &a, const double2 *ptab,
+                                   hipStream_t stream)
+{
+    return first ? launch_tile_t<R, A_LOG, QT, true, true, false>(sv, a, ptab, stream)
+                 : launch_tile_t<R, A_LOG, QT, true, false, false>(sv, a, ptab, stream);
+}
+
+// Gated passes, with or without the cost phase.
+template <typename R, int A_LOG, int QT>
+static hipError_t launch_gated(bool cost, bool first, void *sv, const TileArgs &a, const double2 *ptab,
+                               hipStream_t stream)
 {
     if (!cost)
         return launch_tile_t<R, A_LOG, QT, false, false, true>(sv, a, ptab, stream);
-    if (first)
-        return gates ? launch_tile_t<R, A_LOG, QT, true, true, true>(sv, a, ptab, stream)
-                     : launch_tile_t<R, A_LOG, QT, true, true, false>(sv, a, ptab, stream);
-    return gates ? launch_tile_t<R, A_LOG, QT, true, false, true>(sv, a, ptab, stream)
-                 : launch_tile_t<R, A_LOG, QT, true, false, false>(sv, a, ptab, stream);
+    return first ? launch_tile_t<R, A_LOG, QT, true, true, true>(sv, a, ptab, stream)
+                 : launch_tile_t<R, A_LOG, QT, true, false, true>(sv, a, ptab, stream);
 }
 
 template <typename R>
 static hipError_t launch_tile_r(int a_log, bool cost, bool first, bool gates, void *sv,
                                 const TileArgs &a, const double2 *ptab, hipStream_t stream)
 {
-    // compile-time tile widths for the full-width tiles of the fused and
-    // cost passes; anything else takes the generic instance
-    if (sizeof(R) == 8 && a_log == 3 && a.Q == 13 && gates)
-        return launch_tile_flags<R, 3, 13>(cost, first, gates, sv, a, ptab, stream);
-    if (sizeof(R) == 8 && a_log == 3 && a.Q == 12 && gates)
-        return launch_tile_flags<R, 3, 12>(cost, first, gates, sv, a, ptab, stream);
-    if (sizeof(R) == 4 && a_log == 4 && a.Q == 14 && gates)
-        return launch_tile_flags<R, 4, 14>(cost, first, gates, sv, a, ptab, stream);
-    if (sizeof(R) == 4 && a_log == 4 && a.Q == 13 && gates)
-        return launch_tile_flags<R, 4, 13>(cost, first, gates, sv, a, ptab, stream);
-    if (a_log == 4 && a.Q == 12 && !gates)
-        return launch_tile_flags<R, 4, 12>(cost, first, gates, sv, a, ptab, stream);
-    switch (a_log) {
-        case 1: return launch_tile_flags<R, 1, 0>(cost, first, gates, sv, a, ptab, stream);
-        case 2: return launch_tile_flags<R, 2, 0>(cost, first, gates, sv, a, ptab, stream);
-        case 3: return launch_tile_flags<R, 3, 0>(cost, first, gates, sv, a, ptab, stream);
-        case 4: return launch_tile_flags<R, 4, 0>(cost, first, gates, sv, a, ptab, stream);
-        default: return hipErrorInvalidValue;
+    if (!gates) {
+        if (a_log == 4 && a.Q == 12)
+            return launch_cost_only<R, 4, 12>(first, sv, a, ptab, stream);
+        switch (a_log) {
+            case 1: return launch_cost_only<R, 1, 0>(first, sv, a, ptab, stream);
+            case 2: return launch_cost_only<R, 2, 0>(first, sv, a, ptab, stream);
+            case 3: return launch_cost_only<R, 3, 0>(first, sv, a, ptab, stream);
+            case 4: return launch_cost_only<R, 4, 0>(first, sv, a, ptab, stream);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    // compile-time tile widths for the full-width tiles of the fused passes;
+    // anything else takes the generic instance.  Register bits per thread:
+    // up to 3 for complex128, 4 for complex64.
+    if constexpr (sizeof(R) == 8) {
+        if (a_log == 3 && a.Q == 13)
+            return launch_gated<R, 3, 13>(cost, first, sv, a, ptab, stream);
+        if (a_log == 3 && a.Q == 12)
+            return launch_gated<R, 3, 12>(cost, first, sv, a, ptab, stream);
+        switch (a_log) {
+            case 1: return launch_gated<R, 1, 0>(cost, first, sv, a, ptab, stream);
+            case 2: return launch_gated<R, 2, 0>(cost, first, sv, a, ptab, stream);
+            case 3: return launch_gated<R, 3, 0>(cost, first, sv, a, ptab, stream);
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        if (a_log == 4 && a.Q == 14)
+            return launch_gated<R, 4, 14>(cost, first, sv, a, ptab, stream);
+        if (a_log == 4 && a.Q == 13)
+            return launch_gated<R, 4, 13>(cost, first, sv, a, ptab, stream);
+        switch (a_log) {
+            case 1: return launch_gated<R, 1, 0>(cost, first, sv, a, ptab, stream);
+            case 2: return launch_gated<R, 2, 0>(cost, first, sv, a, ptab, stream);
+            case 3: return launch_gated<R, 3, 0>(cost, first, sv, a, ptab, stream);
+            case 4: return launch_gated<R, 4, 0>(cost, first, sv, a, ptab, stream);
+            default: return hipErrorInvalidValue;
+        }
     }
 }
 
@@ -779,7 +927,7 @@ hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, 
 
 size_t phase_table_entries(const TileArgs &a, int a_log)
 {
-    return ((size_t)1 << a.nh) * (size_t)(a.Q + 1) + ((size_t)(1 + a_log) << a.nthr) + 16;
+    return (size_t)run_tiles(a) * (size_t)(a.Q + 1) + ((size_t)(1 + a_log) << a.nthr) + 16;
 }
 
 hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, hipStream_t stream)
@@ -819,7 +967,7 @@ static hipError_t launch_expect_t(const void *sv, const TileArgs &a, double *par
                                   hipStream_t stream)
 {
     typedef typename CT<R>::T T;
-    const uint64_t tiles = 1ull << a.nh;
+    const uint64_t tiles = run_tiles(a);
     hipLaunchKernelGGL((k_expect<R, A_LOG>), dim3((unsigned)tiles), dim3(1u << a.nthr), 0, stream,
                        reinterpret_cast<const T *>(sv), a, partials);
     hipError_t e = hipGetLastError();

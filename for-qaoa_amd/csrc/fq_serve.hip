@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_slab_phase_table(const TileArgs a, int 
     const PlanDev &p = sp.plan;
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t q1 = (uint64_t)a.Q + 1;
-    const uint64_t ntile_e = (1ull << a.nh) * q1;
+    const uint64_t ntile_e = run_tiles(a) * q1;
     const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
     double angle;
     if (gid < ntile_e) {
@@ -119,12 +119,17 @@ __global__ void __launch_bounds__(256) k_slab_phase_table(const TileArgs a, int 
 
 // One tile pass over the slab.  a describes the virtual state (N + s qubits;
 // the slot bits are the top high bits); tile index >> slot_shift is the slot.
-template <typename R, int A_LOG, bool COST>
+// MIRROR (half-state slabs, slots of 2^(N-1) amplitudes with z_{N-1} = 0):
+// the top register bit is qubit N-1; tile tau (top local bit 0) is paired with
+// tau ^ local_mask of the same slot at complemented element offsets, so RX on
+// that bit is the butterfly psi(b) <-> psi(~b).
+template <typename R, int A_LOG, bool COST, bool MIRROR>
 __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restrict__ sv, const TileArgs a,
                                                     const SlotParams *__restrict__ slots, int slot_shift)
 {
     typedef typename CTS<R>::T T;
     constexpr int A = 1 << A_LOG;
+    constexpr int AH = MIRROR ? A / 2 : A;
     extern __shared__ __align__(16) unsigned char fq_smem_s[];
     const int Q = a.Q, nthr = a.nthr;
     const int t = threadIdx.x;
@@ -136,7 +141,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
     for (int j = 0; j < A; ++j) {
         uint64_t o = 0;
 #pragma unroll
-        for (int r = 0; r < A_LOG; ++r)
+        for (int r = 0; r < (MIRROR ? A_LOG - 1 : A_LOG); ++r)
             if ((j >> r) & 1)
                 o |= 1ull << a.rpos[r];
         roff[j] = o;
@@ -151,13 +156,19 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
         if (!sp.active)
             continue;  // uniform over the work-group
         const uint64_t lt = tile & local_mask;
+        if (MIRROR && ((lt >> (slot_shift - 1)) & 1ull))
+            continue;  // handled with its partner tile
         T *const p0 = sv + (unpack_runs_s(tile, a.hruns) + toff);
+        T *const pm = MIRROR ? sv + (unpack_runs_s(tile ^ local_mask, a.hruns) + a.mirror_pm - toff) : p0;
         T v[A];
         const bool first = COST && sp.cost_first;
         if (!first) {
 #pragma unroll
-            for (int j = 0; j < A; ++j)
+            for (int j = 0; j < AH; ++j)
                 v[j] = p0[roff[j]];
+#pragma unroll
+            for (int j = AH; j < A; ++j)
+                v[j] = pm[-(int64_t)roff[j - AH]];
         }
         if (COST) {
             const double2 *f_t = sp.ptab + lt * q1;
@@ -263,8 +274,11 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
                 v[j] = lds[nat + lds_pad_s(j << nthr)];
         }
 #pragma unroll
-        for (int j = 0; j < A; ++j)
+        for (int j = 0; j < AH; ++j)
             p0[roff[j]] = v[j];
+#pragma unroll
+        for (int j = AH; j < A; ++j)
+            pm[-(int64_t)roff[j - AH]] = v[j];
     }
 }
 
@@ -410,13 +424,13 @@ hipError_t launch_slab_phase_table(const TileArgs &local, int a_log, const SlotP
     return hipGetLastError();
 }
 
-template <typename R, int A_LOG, bool COST>
+template <typename R, int A_LOG, bool COST, bool MIRROR>
 static hipError_t launch_slab_pass_t(void *sv, const TileArgs &a, const SlotParams *slots, int slot_shift,
                                      hipStream_t stream)
 {
     typedef typename CTS<R>::T T;
     const size_t lds = a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
-    const void *fn = reinterpret_cast<const void *>(&k_slab_pass<R, A_LOG, COST>);
+    const void *fn = reinterpret_cast<const void *>(&k_slab_pass<R, A_LOG, COST, MIRROR>);
     if (lds > 65536) {
         static bool attr_set = false;
         if (!attr_set) {
@@ -435,17 +449,20 @@ static hipError_t launch_slab_pass_t(void *sv, const TileArgs &a, const SlotPara
     const uint64_t tiles = 1ull << a.nh;
     const uint64_t resident = (uint64_t)per_cu * cus_s();
     const uint64_t grid = tiles < resident ? tiles : resident;
-    hipLaunchKernelGGL((k_slab_pass<R, A_LOG, COST>), dim3((unsigned)grid), dim3(threads), lds, stream,
+    hipLaunchKernelGGL((k_slab_pass<R, A_LOG, COST, MIRROR>), dim3((unsigned)grid), dim3(threads), lds, stream,
                        reinterpret_cast<T *>(sv), a, slots, slot_shift);
     return hipGetLastError();
 }
 
-hipError_t launch_slab_pass(int dtype, int a_log, bool cost, void *sv, const TileArgs &a,
+hipError_t launch_slab_pass(int dtype, int a_log, bool cost, bool mirror, void *sv, const TileArgs &a,
                             const SlotParams *slots, int slot_shift, hipStream_t stream)
 {
+    if (mirror && (cost || slot_shift < 1))
+        return hipErrorInvalidValue;
 #define FQ_SP(R, AL) \
-    return cost ? launch_slab_pass_t<R, AL, true>(sv, a, slots, slot_shift, stream) \
-                : launch_slab_pass_t<R, AL, false>(sv, a, slots, slot_shift, stream);
+    return mirror ? launch_slab_pass_t<R, AL, false, true>(sv, a, slots, slot_shift, stream) \
+           : cost ? launch_slab_pass_t<R, AL, true, false>(sv, a, slots, slot_shift, stream) \
+                  : launch_slab_pass_t<R, AL, false, false>(sv, a, slots, slot_shift, stream);
     if (dtype == 0) {
         switch (a_log) {
             case 1: FQ_SP(double, 1)

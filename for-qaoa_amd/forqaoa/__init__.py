@@ -42,6 +42,7 @@ from ._lib import (
     FQ_K_TOPK,
     FQ_SCHED_FUSED,
     FQ_SCHED_REFERENCE,
+    FQ_SCHED_Z2,
     FQError,
     KERNEL_CLASSES,
     check,
@@ -49,8 +50,9 @@ from ._lib import (
 )
 
 DTYPES = {"c128": FQ_C128, "c64": FQ_C64, FQ_C128: FQ_C128, FQ_C64: FQ_C64}
-SCHEDULES = {"reference": FQ_SCHED_REFERENCE, "fused": FQ_SCHED_FUSED,
-             FQ_SCHED_REFERENCE: FQ_SCHED_REFERENCE, FQ_SCHED_FUSED: FQ_SCHED_FUSED}
+SCHEDULES = {"reference": FQ_SCHED_REFERENCE, "fused": FQ_SCHED_FUSED, "z2": FQ_SCHED_Z2,
+             FQ_SCHED_REFERENCE: FQ_SCHED_REFERENCE, FQ_SCHED_FUSED: FQ_SCHED_FUSED,
+             FQ_SCHED_Z2: FQ_SCHED_Z2}
 
 
 def _dptr(a):

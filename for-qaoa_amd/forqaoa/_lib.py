@@ -24,6 +24,7 @@ FQ_C64 = 1
 
 FQ_SCHED_REFERENCE = 0
 FQ_SCHED_FUSED = 1
+FQ_SCHED_Z2 = 2
 
 FQ_K_COST, FQ_K_RX, FQ_K_SWAP, FQ_K_FUSED, FQ_K_REDUCE, FQ_K_TOPK, FQ_K_ADJOINT = range(7)
 KERNEL_CLASSES = ["cost", "rx", "swap", "fused", "reduce", "topk", "adjoint"]
@@ -87,6 +88,7 @@ SIGNATURES = {
     "fq_set_memory_budget": (_i, [_sz]),
     "fq_memory_in_use": (_i, [_szp]),
     "fq_slab_create": (_i, [_i, _i, _i, ctypes.POINTER(_vp)]),
+    "fq_slab_create_sched": (_i, [_i, _i, _i, _i, ctypes.POINTER(_vp)]),
     "fq_slab_destroy": (_i, [_vp]),
     "fq_slab_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "fq_slab_assign": (_i, [_vp, _i, _vp, _i, _dp, _dp]),

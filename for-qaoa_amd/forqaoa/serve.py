@@ -34,7 +34,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import FQ_C64, FQ_C128, DTYPES, FQError, Graph, State, _lib
+from . import FQ_C64, FQ_C128, DTYPES, SCHEDULES, FQError, Graph, State, _lib
 from ._lib import check, lib
 
 
@@ -98,10 +98,15 @@ def default_slots(N):
 
 
 class Slab:
-    def __init__(self, N, nslots, dtype):
+    """K packed slots of one N; schedule "fused" (full states) or "z2" (each
+    slot stores the half z_{N-1} = 0 of its state, psi(b) = psi(~b))."""
+
+    def __init__(self, N, nslots, dtype, schedule="fused"):
         h = ctypes.c_void_p()
-        check(lib().fq_slab_create(int(N), int(nslots), DTYPES[dtype], ctypes.byref(h)))
+        check(lib().fq_slab_create_sched(int(N), int(nslots), DTYPES[dtype], SCHEDULES[schedule],
+                                         ctypes.byref(h)))
         self.h = h
+        self.schedule = schedule
         self.N = N
         self.K = nslots
         self.free = list(range(nslots))
@@ -163,9 +168,12 @@ class Slab:
 
 
 class Server:
-    def __init__(self, dtype="c64", budget_bytes=200e9, slots_for=default_slots):
+    """schedule: "z2" (half-state slabs, the default for N >= 3) or "fused"."""
+
+    def __init__(self, dtype="c64", budget_bytes=200e9, slots_for=default_slots, schedule="z2"):
         self.dtype = dtype
         self.slots_for = slots_for
+        self.schedule = schedule
         check(lib().fq_set_memory_budget(ctypes.c_size_t(int(budget_bytes))))
         self.slabs = {}
         self.graphs = {}
@@ -182,7 +190,8 @@ class Server:
         sb = self.slabs.get(N)
         while sb is None:
             try:
-                sb = Slab(N, self.slots_for(N), self.dtype)
+                sched = self.schedule if N >= 3 else "fused"
+                sb = Slab(N, self.slots_for(N), self.dtype, sched)
             except FQError as e:
                 if e.code != _lib.FQ_ERR_BUDGET:
                     raise
@@ -272,17 +281,22 @@ def latency_stats(done):
                 mean_ms=float(lat.mean() * 1e3))
 
 
-def bytes_floor(stream, requests, dtype="c64"):
-    """Algorithmic HBM bytes of the fused schedule for these requests (layer 0
-    writes once, every other pass reads and writes; plus one <C> read)."""
+def bytes_floor(stream, requests, dtype="c64", schedule="fused"):
+    """Algorithmic HBM bytes of the slab schedule for these requests (layer 0
+    writes once, every other pass reads and writes; plus one <C> read).  The
+    half-state schedule stores and moves half of each state; its tile passes
+    cover N qubits with the first pass limited to N-1."""
     elem = 16 if DTYPES[dtype] == FQ_C128 else 8
     qmax = 13 if DTYPES[dtype] == FQ_C128 else 14
-    cmin = 3 if DTYPES[dtype] == FQ_C128 else 4
+    carriers = 3 if DTYPES[dtype] == FQ_C128 else 4
+    z2 = schedule == "z2"
     tot = 0.0
     for r in requests:
-        S = (1 << r.N) * elem
-        q = min(qmax, r.N)
+        half = z2 and r.N >= 3
+        S = (1 << (r.N - 1 if half else r.N)) * elem
+        q = min(qmax, r.N - 1 if half else r.N)
         rest = r.N - q
-        npass = 1 + (0 if rest <= 0 else -(-rest // (q - cmin)))
+        per = q - min(carriers, q - 1)
+        npass = 1 + (0 if rest <= 0 else -(-rest // per))
         tot += S * (2 * npass * r.P - 1) + S
     return tot

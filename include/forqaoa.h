@@ -49,6 +49,9 @@ extern "C" {
 /* layer schedules for fq_run */
 #define FQ_SCHED_REFERENCE 0 /* exactly src/Weighted.cpp:115-137: cost, R x RX chunk, R-1 x transposition */
 #define FQ_SCHED_FUSED 1     /* cost fused into the first tile pass; high qubits by strided tiles, no transposition */
+#define FQ_SCHED_Z2 2        /* FQ_SCHED_FUSED on the half state z_{N-1} = 0 (psi(b) = psi(~b) for Max-Cut
+                                QAOA from |+>); RX on the top qubit pairs complementary tiles; the full
+                                state is written back after the last layer.  N >= 3. */
 
 /* kernel classes for timing */
 #define FQ_K_COST 0
@@ -125,7 +128,7 @@ int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_swap);
 /* The driver loop (src/Weighted.cpp:97-137) for P layers with per-layer
  * angles gammas[p], betas[p].  FQ_SCHED_REFERENCE also relabels g after each
  * transposition exactly as the driver does (g ends in the state's physical
- * labels); FQ_SCHED_FUSED leaves the layout and g unchanged. */
+ * labels); FQ_SCHED_FUSED and FQ_SCHED_Z2 leave the layout and g unchanged. */
 int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
            int schedule);
 
@@ -184,6 +187,10 @@ int fq_memory_in_use(size_t *bytes);
  * request.  Calls on one slab run in order on the slab's own HIP stream. */
 typedef struct fq_slab fq_slab;
 int fq_slab_create(int N, int nslots, int dtype, fq_slab **out);
+/* As fq_slab_create with a layer schedule: FQ_SCHED_FUSED (what
+ * fq_slab_create uses) or FQ_SCHED_Z2 (each slot holds the half state
+ * z_{N-1} = 0: half the memory and traffic; N >= 3). */
+int fq_slab_create_sched(int N, int nslots, int dtype, int schedule, fq_slab **out);
 int fq_slab_destroy(fq_slab *sb);
 int fq_slab_info(const fq_slab *sb, int *N, int *nslots, int *free_slots, int *active);
 int fq_slab_assign(fq_slab *sb, int slot, fq_graph *g, int P, const double *gammas,

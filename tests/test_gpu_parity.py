@@ -257,3 +257,83 @@ def test_n30_reference_and_fused_schedules_agree():
     vb = b.gather(idx, logical=True)
     assert ea == pytest.approx(eb, rel=1e-12)
     assert np.max(np.abs(va - vb)) < 1e-12 * 2.0 ** (-N / 2) * 10
+
+
+# ---- half-state (Z2) schedule: psi(b) = psi(~b) --------------------------------
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("name", ["n20_gnp05", "s10_gnp", "s9_gnp_c4", "s5_gnp_c5"])
+def test_run_z2_schedule_vs_oracle(dtype, name):
+    d = GU.instances()[name]
+    N, C = d["N"], d["C"]
+    sv, qmap, W = O.run_weighted(N, C, d["edges"], d["gammas"], d["betas"])
+    st = State(N, C=C, dtype=dtype)
+    g = Graph(N, d["edges"])
+    st.run(g, d["gammas"], d["betas"], "z2")
+    assert list(st.qubit_map()) == list(range(N))
+    got = st.copy_out(logical=True)
+    assert rel_err(got, O.to_logical(sv, qmap)) < TOL[dtype]
+    ex, _ = st.expect_cut(g)
+    assert ex == pytest.approx(O.expect_cut(sv, W), rel=TOL[dtype])
+
+
+@pytest.mark.parametrize("name", ["s4_gnp", "s6_reg3", "s7_gnp_int", "s8_gnp", "s10_reg4_c10"])
+def test_run_z2_vs_dense_fixtures(name):
+    d = GU.instances()[name]
+    psi = GU.dense_small()[name]
+    st = State(d["N"], C=d["C"])
+    g = Graph(d["N"], d["edges"])
+    st.run(g, d["gammas"], d["betas"], "z2")
+    assert rel_err(st.copy_out(logical=True), psi) < 1e-12
+
+
+@pytest.mark.parametrize("N", [3, 4, 5, 6, 13, 14, 15, 16, 17, 23])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_z2_sizes_around_tile_limits(N, dtype):
+    # one mirror pass right after the contiguous pass (N <= Qmax + 1), and
+    # several strided passes before it
+    edges = G.gnp(N, 0.5, 70 + N, "uniform")
+    g_, b_ = G.angles(3, 80 + N)
+    C = min(N, 13 if dtype == "c128" else 14)
+    sv, qmap, W = O.run_weighted(N, C, edges, g_, b_)
+    st = State(N, C=C, dtype=dtype)
+    g = Graph(N, edges)
+    st.run(g, g_, b_, "z2")
+    assert rel_err(st.copy_out(logical=True), O.to_logical(sv, qmap)) < TOL[dtype]
+
+
+def test_z2_rejects_tiny_states():
+    from forqaoa import FQError
+    st = State(2, C=2)
+    g = Graph(2, [(0, 1, 1.0)])
+    with pytest.raises(FQError):
+        st.run(g, [0.3], [0.2], "z2")
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_n30_z2_closed_form_and_fused_agree(dtype):
+    c = _kn(30, "random P=5")
+    N = 30
+    st = State(N, C=13 if dtype == "c128" else 14, dtype=dtype)
+    g = Graph(N, G.complete(N))
+    st.run(g, c["gammas"], c["betas"], "z2")
+    rng = np.random.default_rng(12)
+    idx = rng.integers(0, 1 << N, size=8192, dtype=np.uint64)
+    got = st.gather(idx, logical=True)
+    want = c["phi"][GU.popcount(idx)]
+    scale = 2.0 ** (-N / 2)
+    assert np.max(np.abs(got - want)) / scale < TOL[dtype] * 10
+    ex, n2 = st.expect_cut(g)
+    assert ex == pytest.approx(c["expect_cut"], rel=TOL[dtype])
+    assert n2 == pytest.approx(1.0, abs=TOL[dtype] * 10)
+    # a non-symmetric-weight instance: z2 against fused
+    d = GU.instances()["n30_gnp05"]
+    g2 = Graph(N, d["edges"])
+    st.run(g2, d["gammas"], d["betas"], "z2")
+    e_z2, _ = st.expect_cut(g2)
+    v_z2 = st.gather(idx, logical=True)
+    st.run(g2, d["gammas"], d["betas"], "fused")
+    e_f, _ = st.expect_cut(g2)
+    v_f = st.gather(idx, logical=True)
+    assert e_z2 == pytest.approx(e_f, rel=TOL[dtype])
+    assert np.max(np.abs(v_z2 - v_f)) / scale < TOL[dtype] * 10

@@ -29,10 +29,11 @@ def solo(N, edges, g, b, dtype):
     return e, amps
 
 
+@pytest.mark.parametrize("schedule", ["fused", "z2"])
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
-def test_slab_slots_match_solo_runs(dtype):
-    N = 15
-    sb = Slab(N, 8, dtype)
+@pytest.mark.parametrize("N", [15, 21])
+def test_slab_slots_match_solo_runs(dtype, schedule, N):
+    sb = Slab(N, 8, dtype, schedule)
     jobs = []
     for k in range(5):
         edges = G.gnp(N, 0.3 + 0.1 * k, 500 + k, "uniform")
@@ -60,10 +61,11 @@ def test_slab_slots_match_solo_runs(dtype):
     sb.destroy()
 
 
+@pytest.mark.parametrize("schedule", ["z2", "fused"])
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
-def test_server_results_equal_solo(dtype):
-    stream = make_stream(60, seed=11, n_range=(12, 18), p_range=(1, 4), rate=None)
-    srv = Server(dtype=dtype, budget_bytes=8e9)
+def test_server_results_equal_solo(dtype, schedule):
+    stream = make_stream(60, seed=11, n_range=(3, 18), p_range=(1, 4), rate=None)
+    srv = Server(dtype=dtype, budget_bytes=8e9, schedule=schedule)
     done = srv.run(stream, realtime=False)
     assert len(done) == 60
     worst = 0.0
@@ -72,7 +74,7 @@ def test_server_results_equal_solo(dtype):
         e_solo, _ = solo(N, edges, r.gammas, r.betas, dtype)
         worst = max(worst, abs(r.expect - e_solo) / abs(e_solo))
         assert r.norm2 == pytest.approx(1.0, abs=1e-4 if dtype == "c64" else 1e-11)
-    print(f"server {dtype}: max relative |<C>_mix - <C>_alone| = {worst:.2e}", flush=True)
+    print(f"server {dtype} {schedule}: max relative |<C>_mix - <C>_alone| = {worst:.2e}", flush=True)
     assert worst < TOL[dtype]
     # requests joined slabs that already had requests in flight
     assert any(r.joined_running > 0 for r in done)

@@ -259,3 +259,21 @@ def test_reference_recipe_is_permutation_symmetric():
     for h in range(N + 1):
         vals = sv[pc == h]
         assert np.max(np.abs(vals - vals[0])) < 1e-15
+
+
+@pytest.mark.parametrize("N,C,P,seed", [(9, 4, 3, 41), (12, 5, 4, 42), (14, 13, 2, 43)])
+def test_final_state_is_complement_symmetric(N, C, P, seed):
+    # psi(b) = psi(~b) for Max-Cut QAOA from |+>: Cut(b) = Cut(~b), the mixer
+    # commutes with X^{(x)N} and |+> is X^{(x)N}-invariant.  The half-state
+    # schedule stores z_{N-1} = 0 only and rests on this; the dense
+    # textbook-gate simulator shows it independently of the oracle.
+    edges = G.gnp(N, 0.5, seed, "uniform")
+    g, b = G.angles(P, seed)
+    sv, qmap, W = O.run_weighted(N, C, edges, g, b)
+    lg = O.to_logical(sv, qmap)
+    full = (1 << N) - 1
+    idx = np.arange(1 << N)
+    np.testing.assert_allclose(lg, lg[full ^ idx], rtol=0, atol=1e-15)
+    if N <= 10:
+        psi = R.dense_qaoa(N, edges, g, b)
+        np.testing.assert_allclose(psi, psi[full ^ idx], rtol=0, atol=1e-14)
