@@ -238,14 +238,15 @@ static int geom_runs(const Geom &g)
     return std::max(make_runs(g.hpos, g.nh, r), make_runs(g.tpos, g.nthr, r));
 }
 
-// Tile order of the persistent passes (tile_at); FQ_TILE_ORDER (environment)
-// selects another for experiments.
+// Tile order of the persistent passes (tile_at): 2, one contiguous range of
+// tiles per work-group (measured fastest on MI355X for the contiguous and
+// the strided passes); FQ_TILE_ORDER (environment) selects another.
 static uint32_t tile_order()
 {
     static int order = -1;
     if (order < 0) {
         const char *e = getenv("FQ_TILE_ORDER");
-        order = e ? std::max(0, std::min(3, atoi(e))) : 0;
+        order = e ? std::max(0, std::min(3, atoi(e))) : 2;
     }
     return (uint32_t)order;
 }
@@ -983,21 +984,37 @@ extern "C" int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_sw
     a.out = swap_out;
     a.in = swap_in;
     a.k = k;
+    // out == 0 (the driver's case): pipelined LDS-tiled transpose with tiles
+    // of 2^10 (complex128) / 2^12 (complex64) amplitudes read as runs of
+    // >= 512 B where the blocks allow; one pair of sub-tiles per work-group
+    // for small blocks; otherwise one thread per pair.
+    a.t = -1;
+    if (swap_out == 0) {
+        const int tile_log = st->dtype == FQ_C128 ? 10 : 12;
+        const int t = std::min(k, tile_log / 2);
+        const int m = t == k ? std::max(0, std::min(tile_log - 2 * k, swap_in - k)) : 0;
+        if (2 * t + m == tile_log) {
+            a.pipe = 1;
+            a.t = t;
+            a.m = m;
+        } else if (k >= 3) {
+            a.t = std::min(k, 5);
+        }
+        if (a.t >= 0) {
+            const uint64_t mb = 1ull << (k - a.t);
+            a.npairs = mb * (mb + 1) / 2;
+        }
+    }
+    // rest bits: everything outside both blocks and the tile's rest bits
     bool used[FQ_MAXN] = {false};
     for (int i = 0; i < k; ++i)
         used[swap_out + i] = used[swap_in + i] = true;
+    for (int i = 0; i < a.m; ++i)
+        used[k + i] = true;
     for (int p = 0; p < N; ++p)
         if (!used[p])
             a.rpos[a.nr++] = (uint8_t)p;
-    // out == 0 (the driver's case): coalesced LDS-tiled transpose; otherwise
-    // (or for tiny blocks) one thread per pair.
-    if (swap_out == 0 && k >= 3) {
-        a.t = std::min(k, 5);
-        const uint64_t m = 1ull << (k - a.t);
-        a.npairs = m * (m + 1) / 2;
-    } else {
-        a.t = -1;
-    }
+    a.rest_fast = a.pipe && a.nr > 0 && a.rpos[0] < swap_in;
     const double bytes = 2.0 * (double)st->bytes * (1.0 - std::ldexp(1.0, -k));
     int rc = timed(st, FQ_K_SWAP, bytes, [&] { return launch_swap(st->dtype, st->sv, a, st->stream); });
     if (rc)

@@ -92,8 +92,11 @@ struct SlotParams {
 
 struct SwapArgs {
     int N, out, in, k;
-    int t;                   // tiled: sub-tile log2 side
-    int nr;                  // number of rest bits
+    int t;                   // tiled: sub-tile log2 side (-1: one thread per pair)
+    int m;                   // pipelined: rest bits [k, k+m) inside the tile (t == k)
+    int pipe;                // pipelined persistent kernel (1) or one pair per work-group (0)
+    int rest_fast;           // pipelined: consecutive units step the rest index first
+    int nr;                  // number of rest bits (outside the tile)
     uint8_t rpos[FQ_MAXN];   // rest bit -> physical position
     uint64_t npairs;         // tiled: number of (X <= Y) sub-tile pairs
 };

@@ -604,6 +604,146 @@ __global__ void __launch_bounds__(256) k_swap_tiled(T *__restrict__ sv, const Sw
     }
 }
 
+// Pair index p -> (X <= Y) with p = Y(Y+1)/2 + X.
+__device__ __forceinline__ void pair_of(uint64_t p, uint64_t &X, uint64_t &Y)
+{
+    uint64_t y = (uint64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while (y * (y + 1) / 2 > p)
+        --y;
+    while ((y + 1) * (y + 2) / 2 <= p)
+        ++y;
+    Y = y;
+    X = p - y * (y + 1) / 2;
+}
+
+// Unit u of the pipelined transpose: the two tile origins and whether the
+// pair is on the diagonal (one tile, transposed in place).
+__device__ __forceinline__ void swap_unit(const SwapArgs &a, uint64_t u, uint64_t &pa, uint64_t &pb, bool &diag)
+{
+    uint64_t p, r;
+    if (a.rest_fast) {
+        r = u & ((1ull << a.nr) - 1ull);
+        p = u >> a.nr;
+    } else {
+        p = u % a.npairs;
+        r = u / a.npairs;
+    }
+    uint64_t X, Y;
+    pair_of(p, X, Y);
+    const uint64_t base = deposit_rest(r, a);
+    pa = base + ((Y << a.t) << a.in) + (X << a.t);
+    pb = base + ((X << a.t) << a.in) + (Y << a.t);
+    diag = X == Y;
+}
+
+// out == 0, persistent and pipelined.  A unit is a pair of tiles (X <= Y)
+// of 2^t x 2^m x 2^t amplitudes for one rest index: tile A has block-0 index
+// X*2^t + xc, rest bits [k, k+m) = rl and block-s index Y*2^t + yc; tile B
+// the same with X and Y exchanged.  Each side is read as 2^t runs of 2^(t+m)
+// contiguous amplitudes (m > 0 only when the whole block is in the tile,
+// t == k), exchanged through LDS (row stride 2^(m+t) + 1: conflict-free for
+// the transposed read) and written back to the other tile.  The next unit's
+// loads are issued before the current unit's LDS exchange.  256 threads,
+// E amplitudes per thread per tile.
+template <typename T> struct NativeVec;
+template <> struct NativeVec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct NativeVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+
+// (register arrays of the clang vector type stay in VGPRs across the loop;
+// arrays of HIP_vector_type<double, 2> went to scratch here)
+template <typename T0, int E>
+__global__ void __launch_bounds__(256) k_swap_pipe(T0 *__restrict__ sv0, const SwapArgs a, uint64_t nunits)
+{
+    typedef typename NativeVec<T0>::type T;
+    T *__restrict__ sv = reinterpret_cast<T *>(sv0);
+    extern __shared__ __align__(16) unsigned char fq_smem[];
+    const int t = a.t, m = a.m, s = a.in, k = a.k;
+    const int side = 1 << t;
+    const int R = (side << m) + 1;
+    T *la = reinterpret_cast<T *>(fq_smem);
+    T *lb = la + (size_t)side * R;
+    const int tid = threadIdx.x;
+
+    // per-thread element coordinates (xc fastest, then rl, then yc)
+    uint64_t off_g[E];
+    int off_l[E], off_t[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = tid + e * 256;
+        const int xc = i & (side - 1);
+        const int rl = (i >> t) & ((1 << m) - 1);
+        const int yc = i >> (t + m);
+        off_g[e] = ((uint64_t)yc << s) + ((uint64_t)rl << k) + (uint64_t)xc;
+        off_l[e] = yc * R + rl * side + xc;  // natural slot
+        off_t[e] = xc * R + rl * side + yc;  // transposed slot
+    }
+
+
+    uint64_t u = blockIdx.x;
+    T va[E], vb[E];
+    uint64_t pa = 0, pb = 0;
+    bool diag = true;
+    if (u < nunits) {
+        swap_unit(a, u, pa, pb, diag);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            va[e] = sv[pa + off_g[e]];
+        if (!diag) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                vb[e] = sv[pb + off_g[e]];
+        }
+    }
+    while (u < nunits) {
+        const uint64_t next = u + gridDim.x;
+        uint64_t na = 0, nb = 0;
+        bool ndiag = true;
+        T wa[E], wb[E];
+        if (next < nunits) {
+            swap_unit(a, next, na, nb, ndiag);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                wa[e] = sv[na + off_g[e]];
+            if (!ndiag) {
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    wb[e] = sv[nb + off_g[e]];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            la[off_l[e]] = va[e];
+        if (!diag) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                lb[off_l[e]] = vb[e];
+        }
+        lds_barrier();
+        if (diag) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                sv[pa + off_g[e]] = la[off_t[e]];
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                sv[pa + off_g[e]] = lb[off_t[e]];
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                sv[pb + off_g[e]] = la[off_t[e]];
+        }
+        lds_barrier();  // the LDS reads finish before the next unit's writes
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            va[e] = wa[e];
+            vb[e] = wb[e];
+        }
+        u = next;
+        pa = na;
+        pb = nb;
+        diag = ndiag;
+    }
+}
+
 // Any out: one thread per (rest, x, y); the x < y threads swap their pair.
 template <typename T>
 __global__ void __launch_bounds__(256) k_swap_generic(T *__restrict__ sv, const SwapArgs a,
@@ -938,9 +1078,43 @@ hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, hipStr
     return hipGetLastError();
 }
 
+template <typename T, int E>
+static hipError_t launch_swap_pipe(void *sv, const SwapArgs &a, hipStream_t stream)
+{
+    const int side = 1 << a.t;
+    const size_t lds = 2 * (size_t)side * (((size_t)side << a.m) + 1) * sizeof(T);
+    const void *fn = reinterpret_cast<const void *>(&k_swap_pipe<T, E>);
+    if (lds > 65536) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024 - 1024);
+            if (e != hipSuccess)
+                return e;
+            attr_set = true;
+        }
+    }
+    const uint64_t nunits = a.npairs << a.nr;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const uint64_t resident = (uint64_t)per_cu * device_cus();
+    const uint64_t grid = nunits < resident ? nunits : resident;
+    hipLaunchKernelGGL((k_swap_pipe<T, E>), dim3((unsigned)grid), dim3(256), lds, stream,
+                       reinterpret_cast<T *>(sv), a, nunits);
+    return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_swap_t(void *sv, const SwapArgs &a, hipStream_t stream)
 {
+    if (a.pipe) {
+        // tiles of 2^10 complex128 / 2^12 complex64 amplitudes
+        constexpr int E = sizeof(T) == 16 ? 4 : 16;
+        if ((1 << (2 * a.t + a.m)) != 256 * E)
+            return hipErrorInvalidValue;
+        return launch_swap_pipe<T, E>(sv, a, stream);
+    }
     if (a.t >= 0) {
         const uint64_t nrest = 1ull << a.nr;
         const uint64_t blocks = a.npairs * nrest;

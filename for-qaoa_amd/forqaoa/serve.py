@@ -91,6 +91,13 @@ def make_stream(n_requests, seed=20251207, n_range=(16, 30), p_range=(1, 10), gn
     return st
 
 
+def in_use():
+    """HBM held by this process's states and slabs (fq_memory_in_use)."""
+    used = ctypes.c_size_t()
+    check(lib().fq_memory_in_use(ctypes.byref(used)))
+    return used.value
+
+
 def default_slots(N):
     """Slots per slab: about 2^24 amplitudes per slab for small N (packing),
     two slots for large N."""
@@ -168,12 +175,13 @@ class Slab:
 
 
 class Server:
-    """schedule: "z2" (half-state slabs, the default for N >= 3) or "fused"."""
+    """schedule: "fused" (full-state slabs) or "z2" (half-state slabs for N >= 3)."""
 
-    def __init__(self, dtype="c64", budget_bytes=200e9, slots_for=default_slots, schedule="z2"):
+    def __init__(self, dtype="c64", budget_bytes=200e9, slots_for=default_slots, schedule="fused"):
         self.dtype = dtype
         self.slots_for = slots_for
         self.schedule = schedule
+        self.budget_bytes = budget_bytes
         check(lib().fq_set_memory_budget(ctypes.c_size_t(int(budget_bytes))))
         self.slabs = {}
         self.graphs = {}
@@ -244,6 +252,14 @@ class Server:
                 if sb.slot_req:
                     sb.step()
                     progressed = True
+            idle = not progressed and all(not s.busy and not s.slot_req for s in self.slabs.values())
+            if idle and any(waiting.values()):
+                # nothing runs and nothing can be released: these requests
+                # can never be admitted under the budget
+                blocked = sorted(N for N, q in waiting.items() if q)
+                raise FQError(_lib.FQ_ERR_BUDGET,
+                              f"requests of N={blocked} do not fit the memory budget "
+                              f"({self.budget_bytes:.3g} bytes, {in_use():.3g} in use)")
             if not progressed and incoming and realtime:
                 wait = incoming[0].arrival - (time.perf_counter() - t0)
                 if wait > 0 and all(not s.busy for s in self.slabs.values()):

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import pyoracle as O
-from forqaoa import Graph, _lib
+from forqaoa import FQError, Graph, _lib
 from forqaoa import graphs as G
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -115,3 +115,18 @@ def test_driver_rejects_multi_device_like_reference_exit():
     r = subprocess.run([exe, "1", "20", "0", "10", "10", "--dtype", "c32"], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 2
+
+
+def test_server_reports_requests_that_never_fit():
+    # budget accounting happens before any device call, so this runs on the CPU
+    from forqaoa.serve import Server, make_stream
+    stream = make_stream(3, seed=5, n_range=(20, 20), p_range=(1, 1))
+    srv = Server(dtype="c64", budget_bytes=1 << 20)
+    try:
+        with pytest.raises(FQError) as e:
+            srv.run(stream, realtime=False)
+        assert e.value.code == _lib.FQ_ERR_BUDGET
+        assert "N=[20]" in str(e.value)
+    finally:
+        srv.close()
+        _lib.check(_lib.lib().fq_set_memory_budget(0))

@@ -12,7 +12,7 @@ import pytest
 
 from forqaoa import FQError, Graph, State, _lib
 from forqaoa import graphs as G
-from forqaoa.serve import Server, Slab, latency_stats, make_stream, one_at_a_time
+from forqaoa.serve import Server, Slab, in_use, latency_stats, make_stream, one_at_a_time
 
 pytestmark = pytest.mark.gpu
 
@@ -65,7 +65,8 @@ def test_slab_slots_match_solo_runs(dtype, schedule, N):
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
 def test_server_results_equal_solo(dtype, schedule):
     stream = make_stream(60, seed=11, n_range=(3, 18), p_range=(1, 4), rate=None)
-    srv = Server(dtype=dtype, budget_bytes=8e9, schedule=schedule)
+    # budgets count from what this process already holds
+    srv = Server(dtype=dtype, budget_bytes=in_use() + 8e9, schedule=schedule)
     done = srv.run(stream, realtime=False)
     assert len(done) == 60
     worst = 0.0
@@ -84,7 +85,7 @@ def test_server_results_equal_solo(dtype, schedule):
 def test_server_continuous_admission_realtime():
     # arrivals while earlier requests run: later requests start before earlier ones finish
     stream = make_stream(80, seed=12, n_range=(14, 20), p_range=(2, 8), rate=4000.0)
-    srv = Server(dtype="c64", budget_bytes=8e9)
+    srv = Server(dtype="c64", budget_bytes=in_use() + 8e9)
     done = srv.run(stream, realtime=True)
     assert len(done) == 80
     overlaps = sum(1 for a in done for b in done if a.rid < b.rid and b.start < a.done and a.N == b.N)
@@ -97,18 +98,17 @@ def test_server_continuous_admission_realtime():
 def test_budget_limits_admission_and_server_still_finishes():
     # a budget that holds one slab at a time: slabs are released when idle
     stream = make_stream(24, seed=13, n_range=(16, 19), p_range=(1, 3), rate=None)
-    srv = Server(dtype="c64", budget_bytes=150e6, slots_for=lambda N: 2)
+    base = in_use()
+    srv = Server(dtype="c64", budget_bytes=base + 150e6, slots_for=lambda N: 2)
     done = srv.run(stream, realtime=False)
     assert len(done) == 24
-    used = ctypes.c_size_t()
-    _lib.check(_lib.lib().fq_memory_in_use(ctypes.byref(used)))
-    assert used.value <= 150e6
+    assert in_use() <= base + 150e6
     srv.close()
     _lib.check(_lib.lib().fq_set_memory_budget(0))
 
 
 def test_budget_error_code():
-    _lib.check(_lib.lib().fq_set_memory_budget(1 << 20))
+    _lib.check(_lib.lib().fq_set_memory_budget(in_use() + (1 << 20)))
     try:
         with pytest.raises(FQError) as e:
             State(20, C=10, dtype="c64")
@@ -119,7 +119,7 @@ def test_budget_error_code():
 
 def test_one_at_a_time_baseline_agrees():
     stream = make_stream(6, seed=14, n_range=(12, 14), p_range=(1, 3))
-    srv = Server(dtype="c128", budget_bytes=4e9)
+    srv = Server(dtype="c128", budget_bytes=in_use() + 4e9)
     done = {r.rid: r.expect for r in srv.run(stream, realtime=False)}
     srv.close()
     one_at_a_time(stream, stream.requests, dtype="c128", schedule="reference")
