@@ -148,6 +148,14 @@ def serving_bench(n_sat=1000, n_base=150, n_lat=600, seed=20251207):
     srv.run(sub, realtime=False)
     t_sub = time.perf_counter() - t0
     srv.close()
+    # the same sample through full-state slabs
+    sub_f = type(full)(graphs=full.graphs, requests=[type(r)(r.rid, r.N, r.P, r.graph_key, r.gammas, r.betas)
+                                                     for r in sat.requests[:n_base]])
+    srv = Server(dtype="c64", budget_bytes=200e9, schedule="fused")
+    t0 = time.perf_counter()
+    srv.run(sub_f, realtime=False)
+    t_sub_f = time.perf_counter() - t0
+    srv.close()
 
     rate = 0.8 * evals_s
     lat_stream = make_stream(n_lat, seed=seed + 1, rate=rate)
@@ -155,17 +163,20 @@ def serving_bench(n_sat=1000, n_base=150, n_lat=600, seed=20251207):
     lat_done = srv.run(lat_stream, realtime=True)
     srv.close()
     lat = latency_stats(lat_done)
-    floor_b = bytes_floor(full, sat.requests, "c64", "fused")
+    floor_b = bytes_floor(full, sat.requests, "c64", "z2")
+    floor_f = bytes_floor(full, sat.requests, "c64", "fused")
     return dict(
         workload="10,000-request stream (N 16..30, P 1..10, G(N,p) p in [0.2,0.8], 80% resident-graph "
-                 "re-evaluations), complex64, 200 GB budget, full-state slabs; bounded samples of it",
+                 "re-evaluations), complex64, 200 GB budget, half-state slabs; bounded samples of it",
         evals_per_s=evals_s, saturated_sample=n_sat, saturated_seconds=t_sat,
         baseline_sample=n_base, baseline_evals_per_s=n_base / t_base,
         server_evals_per_s_same_sample=n_base / t_sub,
         speedup_vs_one_at_a_time=(n_base / t_sub) / (n_base / t_base),
         one_at_a_time_fused_evals_per_s=n_base / t_base_f,
+        server_full_state_slabs_evals_per_s_same_sample=n_base / t_sub_f,
         baseline_max_rel_diff=worst,
         bandwidth_floor_evals_per_s=len(sat.requests) / (floor_b / (HBM_PEAK_GBS * 1e9)),
+        bandwidth_floor_full_state_evals_per_s=len(sat.requests) / (floor_f / (HBM_PEAK_GBS * 1e9)),
         latency_rate_per_s=rate, latency_sample=n_lat, **lat)
 
 

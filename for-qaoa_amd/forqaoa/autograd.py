@@ -6,7 +6,8 @@
 The angle tensors are small; they are read on the host (CPU tensors, or
 detached copies of device tensors) and the state vector lives in the
 library's own HBM allocation for the lifetime of the autograd node.  The
-forward pass is fq_run(..., FQ_SCHED_FUSED) + fq_expect_cut; the backward
+forward pass is fq_run(..., FQ_SCHED_Z2) (FQ_SCHED_FUSED for N < 3) +
+fq_expect_cut; the backward
 pass is fq_backward, which walks the state back layer by layer (no
 checkpoints).  The reference has no expectation or gradient path; this is
 the differentiation form named for this build.
@@ -27,7 +28,8 @@ class QAOAExpectation(torch.autograd.Function):
         if g.shape != b.shape or g.ndim != 1:
             raise ValueError("gammas and betas must be 1-D of equal length")
         st = State(graph.N, C=C if C is not None else (13 if dtype == "c128" else 14), dtype=dtype)
-        st.run(graph, g, b, "fused")
+        # the half-state schedule leaves the same identity-layout final state
+        st.run(graph, g, b, "z2" if graph.N >= 3 else "fused")
         e, _ = st.expect_cut(graph)
         ctx.st = st
         ctx.graph = graph

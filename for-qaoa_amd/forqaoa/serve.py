@@ -175,9 +175,10 @@ class Slab:
 
 
 class Server:
-    """schedule: "fused" (full-state slabs) or "z2" (half-state slabs for N >= 3)."""
+    """schedule: "z2" (half-state slabs for N >= 3, the default) or "fused"
+    (full-state slabs)."""
 
-    def __init__(self, dtype="c64", budget_bytes=200e9, slots_for=default_slots, schedule="fused"):
+    def __init__(self, dtype="c64", budget_bytes=200e9, slots_for=default_slots, schedule="z2"):
         self.dtype = dtype
         self.slots_for = slots_for
         self.schedule = schedule

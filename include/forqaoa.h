@@ -162,7 +162,7 @@ int fq_sync(fq_state *st);
 
 /* d<C>/dgamma_k and d<C>/dbeta_k of the P-layer evaluation by reverse
  * uncompute (no checkpoints): st must hold the state fq_run(st, g, P, gammas,
- * betas, FQ_SCHED_FUSED) produced (identity qubit map).  The state is consumed
+ * betas, FQ_SCHED_FUSED or FQ_SCHED_Z2) produced (identity qubit map).  The state is consumed
  * (walked back to the launch-control state); a second state-sized buffer is
  * allocated on first use.  expect (may be NULL) receives <C>.  FP64
  * accumulation, fixed reduction order.  Waits. */
