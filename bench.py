@@ -180,6 +180,42 @@ def serving_bench(n_sat=1000, n_base=150, n_lat=600, seed=20251207):
         latency_rate_per_s=rate, latency_sample=n_lat, **lat)
 
 
+def adjoint_bench(n_sets=16, seed=2026):
+    """The adjoint workload: N=26, P=8, seeded 4-regular graph, complex64;
+    for each of 16 random angle sets one forward evaluation (half-state
+    schedule) and one backward pass (fq_backward).  Reports gradients/s and
+    the reverse pass's achieved bandwidth (4S per pair pass)."""
+    import numpy as np
+    from forqaoa import FQ_K_ADJOINT, FQ_K_FUSED, Graph, State
+    from forqaoa import graphs as G
+
+    w = G.workload("n26_reg4")
+    N, P = w["N"], w["P"]
+    st = State(N, C=14, dtype="c64")
+    g = Graph(N, w["edges"])
+    rng = np.random.default_rng(seed)
+    sets = [(rng.uniform(0, np.pi, P), rng.uniform(0, np.pi, P)) for _ in range(n_sets)]
+    st.run(g, *sets[0], "z2")
+    st.backward(g, *sets[0])  # warm-up (allocates the adjoint buffer)
+    st.timing(True)
+    st.timing_reset()
+    t0 = time.perf_counter()
+    for gm, bt in sets:
+        st.run(g, gm, bt, "z2")
+        st.backward(g, gm, bt)
+    st.sync()
+    secs = time.perf_counter() - t0
+    ms_a, cnt_a, by_a = st.timing_get(FQ_K_ADJOINT)
+    ms_f, cnt_f, by_f = st.timing_get(FQ_K_FUSED)
+    st.timing(False)
+    st.destroy()
+    g.destroy()
+    return dict(workload="n26_reg4: N=26 P=8 4-regular, complex64, 16 angle sets, forward (half-state) + "
+                         "backward per set", gradients_per_s=n_sets / secs, ms_per_gradient=secs / n_sets * 1e3,
+                adjoint_gbs=by_a / ms_a / 1e6 if ms_a else None, adjoint_ms_per_set=ms_a / n_sets,
+                forward_ms_per_set=ms_f / n_sets)
+
+
 def traffic_from_profiles():
     if not os.path.exists(PROFILE_TRAFFIC):
         return None
@@ -236,6 +272,13 @@ def main():
                 extra.append(r)
             except Exception as e:  # report, keep the main line
                 extra.append(dict(workload=name, dtype=dt, schedule=sch, error=str(e)))
+    adjoint = None
+    if not args.no_extra:
+        try:
+            adjoint = adjoint_bench()
+            log(f"adjoint: {adjoint['gradients_per_s']:.1f} gradients/s, pair pass {adjoint['adjoint_gbs']:.0f} GB/s")
+        except Exception as e:
+            adjoint = dict(error=repr(e))
     serving = None
     if not args.no_serve:
         try:
@@ -283,6 +326,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": main_res["kernels"],
         "extra": extra,
+        "adjoint": adjoint,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -176,7 +176,7 @@ class State:
         check(lib().fq_run(self.handle, graph.handle, len(g), _dptr(g), _dptr(b), SCHEDULES[schedule]))
 
     def backward(self, graph, gammas, betas):
-        """Adjoint gradient of <C> after run(graph, gammas, betas, "fused").
+        """Adjoint gradient of <C> after run(graph, gammas, betas, "fused" or "z2").
 
         Consumes the state.  Returns (<C>, dC/dgammas, dC/dbetas)."""
         g = np.ascontiguousarray(gammas, dtype=np.float64)
