@@ -139,13 +139,39 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
     const double2 *rr_tab = COSTBW ? thr_tab + ((size_t)(1 + A_LOG) << nthr) : nullptr;
 
     double accX = 0.0, accG = 0.0;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t base = unpack_runs_a(tile, a.hruns) + toff;
-        T vp[A], vl[A];
+    // one contiguous range of tiles per work-group; the next tile's
+    // amplitudes load while the current one is in its LDS phases (not in the
+    // complex128 cost pass, whose registers are full without it)
+    constexpr bool PREFETCH = !(COSTBW && sizeof(R) == 8);
+    const uint64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint64_t first_tile = (uint64_t)blockIdx.x * per;
+    const uint64_t end_tile = first_tile + per < ntiles ? first_tile + per : ntiles;
+    T vp[A], vl[A];
+    if (first_tile < end_tile) {
+        const uint64_t b0 = unpack_runs_a(first_tile, a.hruns) + toff;
 #pragma unroll
         for (int j = 0; j < A; ++j) {
-            vp[j] = psi[base + roff[j]];
-            vl[j] = lam[base + roff[j]];
+            vp[j] = psi[b0 + roff[j]];
+            vl[j] = lam[b0 + roff[j]];
+        }
+    }
+    for (uint64_t tile = first_tile; tile < end_tile; ++tile) {
+        const uint64_t base = unpack_runs_a(tile, a.hruns) + toff;
+        T wp[A], wl[A];
+        if (!PREFETCH && tile != first_tile) {
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                vp[j] = psi[base + roff[j]];
+                vl[j] = lam[base + roff[j]];
+            }
+        }
+        if (PREFETCH && tile + 1 < end_tile) {
+            const uint64_t b1 = unpack_runs_a(tile + 1, a.hruns) + toff;
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                wp[j] = psi[b1 + roff[j]];
+                wl[j] = lam[b1 + roff[j]];
+            }
         }
         if (COSTBW) {
             // per-tile cut coefficients (published by the first LDS barrier)
@@ -296,6 +322,13 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
         // cut coefficients are rewritten only after every thread passed the
         // barrier below
         lds_barrier_a();
+        if (PREFETCH) {
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                vp[j] = wp[j];
+                vl[j] = wl[j];
+            }
+        }
     }
     // fixed-order work-group reduction: waves by shuffle, then 16 slots
 #pragma unroll
@@ -415,9 +448,11 @@ size_t pair_lds_bytes(int dtype, int Q)
     return 2 * (((size_t)1 << Q) + ((size_t)1 << Q) / 16) * elem;
 }
 
+// Persistent grid: as many work-groups as can be resident (one partial sum
+// pair each, at most the tile count).
 template <typename R, int A_LOG, bool COSTBW>
 static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const double2 *ptab,
-                                double *partials, unsigned grid, hipStream_t stream)
+                                double *partials, unsigned *grid_out, hipStream_t stream)
 {
     typedef typename CTA<R>::T T;
     const size_t lds = pair_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q);
@@ -432,22 +467,23 @@ static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const d
             attr_set = true;
         }
     }
-    hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW>), dim3(grid), dim3(1u << a.nthr), lds, stream,
+    const unsigned threads = 1u << a.nthr;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const uint64_t tiles = 1ull << a.nh;
+    const uint64_t resident = (uint64_t)per_cu * (uint64_t)cus_a();
+    const unsigned grid = (unsigned)(tiles < resident ? tiles : resident);
+    *grid_out = grid;
+    hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW>), dim3(grid), dim3(threads), lds, stream,
                        reinterpret_cast<T *>(psi), reinterpret_cast<T *>(lam), a, ptab, partials);
     return hipGetLastError();
 }
 
-unsigned pair_grid(const TileArgs &a)
-{
-    const uint64_t tiles = 1ull << a.nh;
-    const uint64_t cus = (uint64_t)cus_a();
-    return (unsigned)(tiles < cus ? tiles : cus);
-}
-
 hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
-                       const double2 *ptab, double *partials, hipStream_t stream)
+                       const double2 *ptab, double *partials, unsigned *grid, hipStream_t stream)
 {
-    const unsigned grid = pair_grid(a);
 #define FQ_PAIR(R, AL)                                                                             \
     return costbw ? launch_pair_t<R, AL, true>(psi, lam, a, ptab, partials, grid, stream)         \
                   : launch_pair_t<R, AL, false>(psi, lam, a, ptab, partials, grid, stream);

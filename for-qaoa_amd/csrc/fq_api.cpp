@@ -1228,11 +1228,12 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
             }
             double *slot = st->d_grad + 2 * (1 + (size_t)k * npass + i);
             int rc = timed(st, FQ_K_ADJOINT, 4.0 * (double)st->bytes, [&] {
+                unsigned grid = 0;
                 hipError_t e = launch_pair(st->dtype, a_log, costbw, st->sv, st->aux, a,
-                                           costbw ? st->d_phase : nullptr, st->d_part, st->stream);
+                                           costbw ? st->d_phase : nullptr, st->d_part, &grid, st->stream);
                 if (e != hipSuccess)
                     return e;
-                return launch_finalize(st->d_part, pair_grid(a), slot, st->stream);
+                return launch_finalize(st->d_part, grid, slot, st->stream);
             });
             if (rc)
                 return rc;

@@ -124,11 +124,11 @@ hipError_t launch_gather(int dtype, const void *sv, const uint64_t *idx, size_t 
 hipError_t launch_fill_zero_one(int dtype, void *sv, int N, hipStream_t stream);
 hipError_t launch_finalize(const double *partials, uint64_t n, double *out2, hipStream_t stream);
 // adjoint (fq_adjoint.hip)
+// grid receives the number of work-groups (= partial sum pairs written).
 hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
-                       const double2 *ptab, double *partials, hipStream_t stream);
+                       const double2 *ptab, double *partials, unsigned *grid, hipStream_t stream);
 hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, const TileArgs &a,
                           double *partials, hipStream_t stream);
-unsigned pair_grid(const TileArgs &a);
 size_t pair_lds_bytes(int dtype, int Q);
 // packed slabs (fq_serve.hip)
 hipError_t launch_slab_phase_table(const TileArgs &local, int a_log, const SlotParams *slots, int nslots,
