@@ -209,6 +209,19 @@ __device__ __forceinline__ uint64_t tile_base(const TileArgs &a, uint64_t tile)
     return unpack_runs(tile, a.hruns);
 }
 
+// Value of v in lane `lane` of the wave (lane uniform).
+__device__ __forceinline__ double2 lane_bcast(double2 v, int lane)
+{
+    const long long xi = __builtin_bit_cast(long long, v.x);
+    const long long yi = __builtin_bit_cast(long long, v.y);
+    const unsigned xl = (unsigned)__builtin_amdgcn_readlane((int)xi, lane);
+    const unsigned xh = (unsigned)__builtin_amdgcn_readlane((int)(xi >> 32), lane);
+    const unsigned yl = (unsigned)__builtin_amdgcn_readlane((int)yi, lane);
+    const unsigned yh = (unsigned)__builtin_amdgcn_readlane((int)(yi >> 32), lane);
+    return make_double2(__builtin_bit_cast(double, ((unsigned long long)xh << 32) | xl),
+                        __builtin_bit_cast(double, ((unsigned long long)yh << 32) | yl));
+}
+
 // deposit the low bits of x into the set bits of mask (q-bit masks)
 template <int QT>
 __device__ __forceinline__ int deposit(int x, unsigned mask, int q)
@@ -297,6 +310,13 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
 
     uint64_t kt = 0;
     uint64_t tile = tile_at(a, ntiles, 0);
+    // compile-time tile width: lane i of each wave holds factor i of the
+    // tile's phase-table row, loaded one tile ahead with the amplitudes
+    constexpr bool ROWPF = COST && QT > 0;
+    const int row_lane = (t & 63) % (QT + 1);
+    double2 fcur = make_double2(0.0, 0.0), fnext = make_double2(0.0, 0.0);
+    if (ROWPF && tile < ntiles)
+        fcur = ptab[tile * q1 + row_lane];
     T v[A];
     if (!FIRST && tile < ntiles) {
         const T *p0 = sv + (tile_base(a, tile) + toff);
@@ -319,13 +339,14 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         // register-register term; all factors come from the phase table.
         if (COST) {
             const double2 *f_t = ptab + tile * q1;
+            auto row = [&](int i) -> double2 { return ROWPF ? lane_bcast(fcur, i) : f_t[i]; };
             const double2 f_thr = thr_tab[t];
             double x = f_thr.x, y = f_thr.y;
-            zmul(x, y, f_t[Q]);
+            zmul(x, y, row(Q));
 #pragma unroll
             for (int i = 0; i < (QT > 0 ? QT - A_LOG : FQ_MAXQ); ++i) {
                 if (QT > 0 || i < nthr) {
-                    const double2 f = f_t[i];
+                    const double2 f = row(i);
                     const double nx = x * f.x - y * f.y;
                     const double ny = x * f.y + y * f.x;
                     const bool on = (t >> i) & 1;
@@ -351,7 +372,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             for (int r = 0; r < A_LOG; ++r) {
                 const double2 fr = thr_tab[((1 + r) << nthr) + t];
                 double fx = fr.x, fy = fr.y;
-                zmul(fx, fy, f_t[nthr + r]);
+                zmul(fx, fy, row(nthr + r));
                 const R cf = (R)fx, sf = (R)fy;
 #pragma unroll
                 for (int j = 0; j < A; ++j)
@@ -367,6 +388,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             }
         }
 
+        if (ROWPF && next < ntiles)
+            fnext = ptab[next * q1 + row_lane];
         T w[A];
         if (!FIRST && next < ntiles) {
             const T *p1 = sv + (tile_base(a, next) + toff);
@@ -456,6 +479,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             for (int j = 0; j < A; ++j)
                 v[j] = w[j];
         }
+        if (ROWPF)
+            fcur = fnext;
         tile = next;
     }
 }

@@ -117,6 +117,41 @@ __global__ void __launch_bounds__(256) k_slab_phase_table(const TileArgs a, int 
     tabs[stride * blockIdx.y + gid] = make_double2(cs, sn);
 }
 
+// First tile at or after tl (below end) that the pass processes: its slot is
+// active and, for a mirror pass, its top local bit is 0.  Skips whole slots.
+template <bool MIRROR>
+__device__ __forceinline__ uint64_t slab_next_live(const SlotParams *__restrict__ slots, uint64_t tl,
+                                                   uint64_t end, int slot_shift)
+{
+    while (tl < end) {
+        const uint64_t slot = tl >> slot_shift;
+        if (!slots[slot].active || (MIRROR && ((tl >> (slot_shift - 1)) & 1ull))) {
+            tl = (slot + 1) << slot_shift;
+            continue;
+        }
+        return tl;
+    }
+    return end;
+}
+
+// The 2^A_LOG amplitudes of one thread in tile tl (mirror: the upper half
+// from the partner tile at complemented offsets).
+template <typename T, int A, int AH, bool MIRROR>
+__device__ __forceinline__ void slab_load(T *v, const T *__restrict__ sv, const TileArgs &a, uint64_t tl,
+                                          uint64_t local_mask, uint64_t toff, const uint64_t *roff)
+{
+    const T *p0 = sv + (unpack_runs_s(tl, a.hruns) + toff);
+#pragma unroll
+    for (int j = 0; j < AH; ++j)
+        v[j] = p0[roff[j]];
+    if (MIRROR) {
+        const T *pm = sv + (unpack_runs_s(tl ^ local_mask, a.hruns) + a.mirror_pm - toff);
+#pragma unroll
+        for (int j = AH; j < A; ++j)
+            v[j] = pm[-(int64_t)roff[j - AH]];
+    }
+}
+
 // One tile pass over the slab.  a describes the virtual state (N + s qubits;
 // the slot bits are the top high bits); tile index >> slot_shift is the slot.
 // MIRROR (half-state slabs, slots of 2^(N-1) amplitudes with z_{N-1} = 0):
@@ -151,25 +186,25 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
     T *lds = reinterpret_cast<T *>(fq_smem_s);
     const int nat = lds_pad_s(t);
 
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // one contiguous range of tiles per work-group; the next live tile's
+    // amplitudes load while the current one is in its LDS phases
+    const uint64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint64_t range0 = (uint64_t)blockIdx.x * per;
+    const uint64_t range_end = range0 + per < ntiles ? range0 + per : ntiles;
+    uint64_t tile = slab_next_live<MIRROR>(slots, range0, range_end, slot_shift);
+    T v[A];
+    if (tile < range_end && !(COST && slots[tile >> slot_shift].cost_first))
+        slab_load<T, A, AH, MIRROR>(v, sv, a, tile, local_mask, toff, roff);
+    while (tile < range_end) {
+        const uint64_t next = slab_next_live<MIRROR>(slots, tile + 1, range_end, slot_shift);
+        T w[A];
+        if (next < range_end && !(COST && slots[next >> slot_shift].cost_first))
+            slab_load<T, A, AH, MIRROR>(w, sv, a, next, local_mask, toff, roff);
         const SlotParams &sp = slots[tile >> slot_shift];
-        if (!sp.active)
-            continue;  // uniform over the work-group
         const uint64_t lt = tile & local_mask;
-        if (MIRROR && ((lt >> (slot_shift - 1)) & 1ull))
-            continue;  // handled with its partner tile
         T *const p0 = sv + (unpack_runs_s(tile, a.hruns) + toff);
         T *const pm = MIRROR ? sv + (unpack_runs_s(tile ^ local_mask, a.hruns) + a.mirror_pm - toff) : p0;
-        T v[A];
         const bool first = COST && sp.cost_first;
-        if (!first) {
-#pragma unroll
-            for (int j = 0; j < AH; ++j)
-                v[j] = p0[roff[j]];
-#pragma unroll
-            for (int j = AH; j < A; ++j)
-                v[j] = pm[-(int64_t)roff[j - AH]];
-        }
         if (COST) {
             const double2 *f_t = sp.ptab + lt * q1;
             const double2 *thr_tab = sp.ptab + (1ull << slot_shift) * q1;
@@ -279,6 +314,10 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
 #pragma unroll
         for (int j = AH; j < A; ++j)
             pm[-(int64_t)roff[j - AH]] = v[j];
+#pragma unroll
+        for (int j = 0; j < A; ++j)
+            v[j] = w[j];
+        tile = next;
     }
 }
 
