@@ -15,49 +15,12 @@
 // uncompute pass; <lambda|S|phi> commutes with the diagonal U and is
 // accumulated in the pass that applies U^+.  FP64 accumulators, fixed-order
 // reductions.
+#include "fq_device.h"
 #include "fq_internal.h"
 
 #include <hip/hip_runtime.h>
 
 namespace fq {
-
-template <typename R> struct CTA;
-template <> struct CTA<double> { typedef double2 T; };
-template <> struct CTA<float> { typedef float2 T; };
-
-__device__ __forceinline__ int lds_pad_a(int e) { return e + (e >> 4); }
-
-__device__ __forceinline__ void lds_barrier_a()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-__device__ __forceinline__ uint64_t unpack_runs_a(uint64_t x, const BitRuns &r)
-{
-    uint64_t o = 0;
-#pragma unroll
-    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
-        if (k < r.n) {
-            const uint32_t d = r.run[k];
-            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
-        }
-    }
-    return o;
-}
-
-template <typename T, typename R>
-__device__ __forceinline__ void rx_pair_a(T &q0, T &q1, R c, R s)
-{
-    T o0, o1;
-    o0.x = q0.x * c + q1.y * s;
-    o0.y = q0.y * c - q1.x * s;
-    o1.x = q0.y * s + q1.x * c;
-    o1.y = -q0.x * s + q1.y * c;
-    q0 = o0;
-    q1 = o1;
-}
 
 // Im(conj(l0) p1 + conj(l1) p0): the pair's contribution to Im<lambda|X_j|psi>
 template <typename T>
@@ -79,21 +42,14 @@ __device__ __forceinline__ void rx_both(T *vp, T *vl, unsigned mask, R c, R s, d
                 if (!((j >> r) & 1)) {
                     const int j1 = j | (1 << r);
                     acc += im_x_pair(vl[j], vl[j1], vp[j], vp[j1]);
-                    rx_pair_a<T, R>(vp[j], vp[j1], c, s);
-                    rx_pair_a<T, R>(vl[j], vl[j1], c, s);
+                    rx_pair<T, R>(vp[j], vp[j1], c, s);
+                    rx_pair<T, R>(vl[j], vl[j1], c, s);
                 }
             }
         }
     }
 }
 
-template <typename T, typename R>
-__device__ __forceinline__ void cmul_a(T &v, R c, R s)
-{
-    const R x = v.x, y = v.y;
-    v.x = x * c - y * s;
-    v.y = x * s + y * c;
-}
 
 // One uncompute pass over a tile geometry for psi and lambda together:
 // RX(rx angle) on the gate bits (accumulating Im<lambda|X_j|psi>), then, if
@@ -101,13 +57,13 @@ __device__ __forceinline__ void cmul_a(T &v, R c, R s)
 // phase from the phase table (built for -gamma) on both.  partials[2*block]
 // and [2*block+1] receive the work-group's two sums.
 template <typename R, int A_LOG, bool COSTBW>
-__global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restrict__ psi,
-                                                    typename CTA<R>::T *__restrict__ lam,
+__global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict__ psi,
+                                                    typename CT<R>::T *__restrict__ lam,
                                                     const TileArgs a,
                                                     const double2 *__restrict__ ptab,
                                                     double *__restrict__ partials)
 {
-    typedef typename CTA<R>::T T;
+    typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
     extern __shared__ __align__(16) unsigned char fq_smem_a[];
     __shared__ double s_c[FQ_MAXQ + 2];
@@ -118,7 +74,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
     const int t = threadIdx.x;
     const int nthreads = blockDim.x;
     const uint64_t ntiles = 1ull << a.nh;
-    const uint64_t toff = unpack_runs_a((uint64_t)t, a.truns);
+    const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     uint64_t roff[A];
 #pragma unroll
     for (int j = 0; j < A; ++j) {
@@ -148,7 +104,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
     const uint64_t end_tile = first_tile + per < ntiles ? first_tile + per : ntiles;
     T vp[A], vl[A];
     if (first_tile < end_tile) {
-        const uint64_t b0 = unpack_runs_a(first_tile, a.hruns) + toff;
+        const uint64_t b0 = unpack_runs(first_tile, a.hruns) + toff;
 #pragma unroll
         for (int j = 0; j < A; ++j) {
             vp[j] = psi[b0 + roff[j]];
@@ -156,7 +112,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
         }
     }
     for (uint64_t tile = first_tile; tile < end_tile; ++tile) {
-        const uint64_t base = unpack_runs_a(tile, a.hruns) + toff;
+        const uint64_t base = unpack_runs(tile, a.hruns) + toff;
         T wp[A], wl[A];
         if (!PREFETCH && tile != first_tile) {
 #pragma unroll
@@ -166,7 +122,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
             }
         }
         if (PREFETCH && tile + 1 < end_tile) {
-            const uint64_t b1 = unpack_runs_a(tile + 1, a.hruns) + toff;
+            const uint64_t b1 = unpack_runs(tile + 1, a.hruns) + toff;
 #pragma unroll
             for (int j = 0; j < A; ++j) {
                 wp[j] = psi[b1 + roff[j]];
@@ -192,15 +148,15 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
             }
         }
         rx_both<T, R, A_LOG>(vp, vl, a.gate_mask >> nthr, c, s, accX);
-        const int nat = lds_pad_a(t);
+        const int nat = lds_pad(t);
         if (a.ngroups > 0) {
 #pragma unroll
             for (int j = 0; j < A; ++j) {
-                ldsP[nat + lds_pad_a(j << nthr)] = vp[j];
-                ldsL[nat + lds_pad_a(j << nthr)] = vl[j];
+                ldsP[nat + lds_pad(j << nthr)] = vp[j];
+                ldsL[nat + lds_pad(j << nthr)] = vl[j];
             }
         }
-        lds_barrier_a();
+        lds_barrier();
         for (int g = 0; g < a.ngroups; ++g) {
             const unsigned gm = a.grp_mask[g];
             const unsigned rest = (unsigned)qmask & ~gm;
@@ -213,7 +169,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
                         ++k;
                     }
             }
-            const int etp = lds_pad_a(et);
+            const int etp = lds_pad(et);
             int pr[A_LOG > 0 ? A_LOG : 1];
             unsigned g2 = gm;
 #pragma unroll
@@ -229,7 +185,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
                 for (int r = 0; r < A_LOG; ++r)
                     if ((j >> r) & 1)
                         e |= 1 << pr[r];
-                ej[j] = lds_pad_a(e);
+                ej[j] = lds_pad(e);
             }
 #pragma unroll
             for (int j = 0; j < A; ++j) {
@@ -242,13 +198,13 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
                 ldsP[etp + ej[j]] = vp[j];
                 ldsL[etp + ej[j]] = vl[j];
             }
-            lds_barrier_a();
+            lds_barrier();
         }
         if (a.ngroups > 0) {
 #pragma unroll
             for (int j = 0; j < A; ++j) {
-                vp[j] = ldsP[nat + lds_pad_a(j << nthr)];
-                vl[j] = ldsL[nat + lds_pad_a(j << nthr)];
+                vp[j] = ldsP[nat + lds_pad(j << nthr)];
+                vl[j] = ldsL[nat + lds_pad(j << nthr)];
             }
         }
         if (COSTBW) {
@@ -288,8 +244,8 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
                 const double sj = 2.0 * cut - a.plan.wtot;
                 // Im(conj(l) * s * p)
                 accG += sj * ((double)vl[j].x * (double)vp[j].y - (double)vl[j].y * (double)vp[j].x);
-                cmul_a(vp[j], (R)x, (R)y);
-                cmul_a(vl[j], (R)x, (R)y);
+                cmul_by(vp[j], (R)x, (R)y);
+                cmul_by(vl[j], (R)x, (R)y);
             }
 #pragma unroll
             for (int r = 0; r < A_LOG; ++r) {
@@ -300,16 +256,16 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
 #pragma unroll
                 for (int j = 0; j < A; ++j)
                     if ((j >> r) & 1) {
-                        cmul_a(vp[j], (R)fx, (R)fy);
-                        cmul_a(vl[j], (R)fx, (R)fy);
+                        cmul_by(vp[j], (R)fx, (R)fy);
+                        cmul_by(vl[j], (R)fx, (R)fy);
                     }
             }
             if (a.has_rr) {
 #pragma unroll
                 for (int j = 0; j < A; ++j) {
                     const double2 f = rr_tab[j];
-                    cmul_a(vp[j], (R)f.x, (R)f.y);
-                    cmul_a(vl[j], (R)f.x, (R)f.y);
+                    cmul_by(vp[j], (R)f.x, (R)f.y);
+                    cmul_by(vl[j], (R)f.x, (R)f.y);
                 }
             }
         }
@@ -321,7 +277,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
         // the next tile's first LDS writes reuse this thread's own slots; the
         // cut coefficients are rewritten only after every thread passed the
         // barrier below
-        lds_barrier_a();
+        lds_barrier();
         if (PREFETCH) {
 #pragma unroll
             for (int j = 0; j < A; ++j) {
@@ -355,11 +311,11 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CTA<R>::T *__restric
 
 // lambda = Cut * psi, and the partial sums of sum |psi|^2 Cut and sum |psi|^2.
 template <typename R, int A_LOG>
-__global__ void __launch_bounds__(256) k_cut_mul(const typename CTA<R>::T *__restrict__ psi,
-                                                 typename CTA<R>::T *__restrict__ lam, const TileArgs a,
+__global__ void __launch_bounds__(256) k_cut_mul(const typename CT<R>::T *__restrict__ psi,
+                                                 typename CT<R>::T *__restrict__ lam, const TileArgs a,
                                                  double *__restrict__ partials)
 {
-    typedef typename CTA<R>::T T;
+    typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
     __shared__ double s_c[FQ_MAXQ + 2];
     __shared__ double s_K;
@@ -368,7 +324,7 @@ __global__ void __launch_bounds__(256) k_cut_mul(const typename CTA<R>::T *__res
     const int t = threadIdx.x;
     const int nthreads = blockDim.x;
     const uint64_t tile = blockIdx.x;
-    const uint64_t base = unpack_runs_a(tile, a.hruns) + unpack_runs_a((uint64_t)t, a.truns);
+    const uint64_t base = unpack_runs(tile, a.hruns) + unpack_runs((uint64_t)t, a.truns);
     for (int i = t; i < a.Q; i += nthreads) {
         double acc = 0.0;
         for (int e = a.plan.csr_off[i]; e < a.plan.csr_off[i + 1]; ++e)
@@ -430,18 +386,6 @@ __global__ void __launch_bounds__(256) k_cut_mul(const typename CTA<R>::T *__res
     }
 }
 
-static int cus_a()
-{
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
-    return cus;
-}
-
 size_t pair_lds_bytes(int dtype, int Q)
 {
     const size_t elem = dtype == 0 ? 16 : 8;
@@ -454,7 +398,7 @@ template <typename R, int A_LOG, bool COSTBW>
 static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const double2 *ptab,
                                 double *partials, unsigned *grid_out, hipStream_t stream)
 {
-    typedef typename CTA<R>::T T;
+    typedef typename CT<R>::T T;
     const size_t lds = pair_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q);
     const void *fn = reinterpret_cast<const void *>(&k_pair_pass<R, A_LOG, COSTBW>);
     if (lds > 65536) {
@@ -468,12 +412,9 @@ static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const d
         }
     }
     const unsigned threads = 1u << a.nthr;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
     const uint64_t tiles = 1ull << a.nh;
-    const uint64_t resident = (uint64_t)per_cu * (uint64_t)cus_a();
+    const uint64_t resident = (uint64_t)per_cu * (uint64_t)device_cu_count();
     const unsigned grid = (unsigned)(tiles < resident ? tiles : resident);
     *grid_out = grid;
     hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW>), dim3(grid), dim3(threads), lds, stream,
@@ -511,8 +452,8 @@ hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, cons
     const dim3 g((unsigned)tiles), b(1u << a.nthr);
 #define FQ_CM(R, AL)                                                                                       \
     hipLaunchKernelGGL((k_cut_mul<R, AL>), g, b, 0, stream,                                               \
-                       reinterpret_cast<const typename CTA<R>::T *>(psi),                                 \
-                       reinterpret_cast<typename CTA<R>::T *>(lam), a, partials);                         \
+                       reinterpret_cast<const typename CT<R>::T *>(psi),                                 \
+                       reinterpret_cast<typename CT<R>::T *>(lam), a, partials);                         \
     return hipGetLastError();
     if (dtype == 0) {
         switch (a_log) {

@@ -83,7 +83,20 @@ extern "C" int fq_version(void) { return 100; }
 struct PlanEntry {
     void *dmem = nullptr;
     PlanDev dev{};
+    void *staging = nullptr;      // pinned copy of the blob (asynchronous upload)
+    hipEvent_t ready = nullptr;   // upload done
+    hipStream_t stream = nullptr; // stream the upload was queued on
 };
+
+// Releases a plan; the caller has synchronized the device.
+static void plan_free(PlanEntry &e)
+{
+    hipFree(e.dmem);
+    hipHostFree(e.staging);
+    if (e.ready)
+        hipEventDestroy(e.ready);
+    e = PlanEntry();
+}
 
 struct fq_graph {
     int N = 0;
@@ -92,6 +105,7 @@ struct fq_graph {
     uint64_t version = 1;       // bumped by every change of W
     uint64_t hashed_version = 0;
     uint64_t w_hash = 0;        // content hash of W at hashed_version
+    uint64_t plan_epoch = 0;    // bumped when the plan cache is released
 };
 
 struct TimingRec {
@@ -328,7 +342,10 @@ static uint64_t hash_bytes(const void *p, size_t n)
 
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-static int get_plan(fq_graph *G, const Geom &g, PlanDev *out)
+// Device cut plan of graph G for tile geometry g, cached per (geometry, W
+// content).  A new plan is uploaded asynchronously on `stream`; a cached plan
+// whose upload was queued on another stream makes `stream` wait for it.
+static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream)
 {
     const int N = G->N;
     if (G->hashed_version != G->version) {
@@ -340,14 +357,25 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out)
     key.append(reinterpret_cast<const char *>(&wh), sizeof(wh));
     auto it = G->plans.find(key);
     if (it != G->plans.end()) {
-        *out = it->second.dev;
+        PlanEntry &e = it->second;
+        if (e.ready && e.stream != stream) {
+            const hipError_t q = hipEventQuery(e.ready);
+            if (q == hipErrorNotReady) {
+                hipGetLastError();
+                FQ_HIP(hipStreamWaitEvent(stream, e.ready, 0));
+            } else if (q != hipSuccess) {
+                return set_err(FQ_ERR_DEVICE, "plan upload failed: %s", hipGetErrorString(q));
+            }
+        }
+        *out = e.dev;
         return FQ_OK;
     }
     if (G->plans.size() >= 64) {
         FQ_HIP(hipDeviceSynchronize());
         for (auto &kv : G->plans)
-            hipFree(kv.second.dmem);
+            plan_free(kv.second);
         G->plans.clear();
+        G->plan_epoch++;
     }
     int ti[FQ_MAXN], hi[FQ_MAXN];
     for (int p = 0; p < N; ++p)
@@ -457,14 +485,23 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out)
     put(o_ch, ch.data(), sizeof(int) * ch.size());
     put(o_hu, hh_u.data(), sizeof(int) * hh_u.size());
     put(o_hv, hh_v.data(), sizeof(int) * hh_v.size());
-    void *d = nullptr;
-    if (hipMalloc(&d, total) != hipSuccess)
+    PlanEntry pe;
+    if (hipMalloc(&pe.dmem, total) != hipSuccess || hipHostMalloc(&pe.staging, total, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&pe.ready, hipEventDisableTiming) != hipSuccess) {
+        hipGetLastError();
+        plan_free(pe);
         return set_err(FQ_ERR_OOM, "plan allocation of %zu bytes failed", total);
-    if (hipMemcpy(d, blob.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
-        hipFree(d);
+    }
+    memcpy(pe.staging, blob.data(), total);
+    pe.stream = stream;
+    if (hipMemcpyAsync(pe.dmem, pe.staging, total, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipEventRecord(pe.ready, stream) != hipSuccess) {
+        hipGetLastError();
+        hipStreamSynchronize(stream);
+        plan_free(pe);
         return set_err(FQ_ERR_DEVICE, "plan upload failed");
     }
-    char *b = (char *)d;
+    char *b = (char *)pe.dmem;
     dev.cH = (const double *)(b + o_cH);
     dev.omega = (const double *)(b + o_om);
     dev.csr_w = (const double *)(b + o_cw);
@@ -474,8 +511,6 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out)
     dev.csr_h = (const int *)(b + o_ch);
     dev.hh_u = (const int *)(b + o_hu);
     dev.hh_v = (const int *)(b + o_hv);
-    PlanEntry pe;
-    pe.dmem = d;
     pe.dev = dev;
     G->plans[key] = pe;
     *out = dev;
@@ -679,7 +714,7 @@ extern "C" int fq_graph_destroy(fq_graph *g)
     if (!g->plans.empty()) {
         hipDeviceSynchronize();
         for (auto &kv : g->plans)
-            hipFree(kv.second.dmem);
+            plan_free(kv.second);
     }
     delete g;
     return FQ_OK;
@@ -773,7 +808,7 @@ static int cost_pass(fq_state *st, fq_graph *g, double gamma, bool first)
     Geom geom = make_geom(st->N, low_tile(Q), a_log);
     TileArgs a;
     geom_to_args(geom, a);
-    if (int rc = get_plan(g, geom, &a.plan))
+    if (int rc = get_plan(g, geom, &a.plan, st->stream))
         return rc;
     set_cost(a, gamma, st->init_amp);
     if (int rc = prepare_phases(st, a, a_log))
@@ -789,17 +824,16 @@ static int cost_pass(fq_state *st, fq_graph *g, double gamma, bool first)
 // writes the full state's upper half, after the last layer).
 enum { Z2_OFF = 0, Z2_LOWER = 1, Z2_MIRROR = 2, Z2_EXPAND = 3 };
 
-// One tile pass with RX on gate positions (and optionally the cost phase).
-static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vector<int> &gates,
-                     double beta, fq_graph *g, double gamma, bool cost, bool first, int kind,
-                     int z2 = Z2_OFF)
+// Launch arguments of one tile pass of an N-qubit state: tile positions
+// (thread bits first), RX gate positions and the half-state mode.
+static int pass_args(int N, int dtype, const std::vector<int> &tile, const std::vector<int> &gates, bool cost,
+                     int z2, TileArgs &a, Geom &geom)
 {
     const int Q = (int)tile.size();
-    const int a_log = tile_a_log(st->dtype, Q);
-    Geom geom = make_geom(st->N, tile, a_log);
+    const int a_log = tile_a_log(dtype, Q);
+    geom = make_geom(N, tile, a_log);
     if (geom_runs(geom) > FQ_MAX_RUNS)
         return set_err(FQ_ERR_ARG, "internal: tile geometry needs more than %d bit runs", FQ_MAX_RUNS);
-    TileArgs a;
     geom_to_args(geom, a);
     uint32_t gm = 0;
     for (int p : gates) {
@@ -816,8 +850,8 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     if (z2 != Z2_OFF) {
         // the lower half is the tiles whose top index bit (qubit N-1) is 0;
         // a mirror pass pairs tile tau with ~tau instead
-        const bool ok = z2 == Z2_LOWER ? geom.nh >= 1 && geom.hpos[geom.nh - 1] == st->N - 1
-                                       : geom.nh >= 1 && tile.back() == st->N - 1 && !cost;
+        const bool ok = z2 == Z2_LOWER ? geom.nh >= 1 && geom.hpos[geom.nh - 1] == N - 1
+                                       : geom.nh >= 1 && tile.back() == N - 1 && !cost;
         if (!ok)
             return set_err(FQ_ERR_ARG, "internal: tile does not fit the half-state pass");
         a.ntiles_run = 1ull << (geom.nh - 1);
@@ -825,12 +859,25 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
             for (int i = 0; i + 1 < Q; ++i)
                 a.mirror_pm += 1ull << tile[i];
     }
+    return FQ_OK;
+}
+
+// One tile pass with RX on gate positions (and optionally the cost phase).
+static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vector<int> &gates,
+                     double beta, fq_graph *g, double gamma, bool cost, bool first, int kind,
+                     int z2 = Z2_OFF)
+{
+    TileArgs a;
+    Geom geom;
+    if (int rc = pass_args(st->N, st->dtype, tile, gates, cost, z2, a, geom))
+        return rc;
+    const int a_log = geom.a_log;
     double s, c;
     sincos(beta / 2, &s, &c);
     a.rx_c = c;
     a.rx_s = s;
     if (cost) {
-        if (int rc = get_plan(g, geom, &a.plan))
+        if (int rc = get_plan(g, geom, &a.plan, st->stream))
             return rc;
         set_cost(a, gamma, st->init_amp);
         if (int rc = prepare_phases(st, a, a_log))
@@ -1119,7 +1166,7 @@ extern "C" int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *
     Geom geom = make_geom(st->N, low_tile(Q), a_log);
     TileArgs a;
     geom_to_args(geom, a);
-    if (int rc = get_plan(g, geom, &a.plan))
+    if (int rc = get_plan(g, geom, &a.plan, st->stream))
         return rc;
     int rc = timed(st, FQ_K_REDUCE, (double)st->bytes, [&] {
         return launch_expect(st->dtype, a_log, st->sv, a, st->d_part, st->d_out2, st->stream);
@@ -1193,7 +1240,7 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
         Geom geom = make_geom(N, low_tile(Q), a_log);
         TileArgs a;
         geom_to_args(geom, a);
-        if (int rc = get_plan(g, geom, &a.plan))
+        if (int rc = get_plan(g, geom, &a.plan, st->stream))
             return rc;
         FQ_HIP(launch_cut_mul(st->dtype, a_log, st->sv, st->aux, a, st->d_part, st->stream));
         FQ_HIP(launch_finalize(st->d_part, 1ull << a.nh, st->d_grad, st->stream));
@@ -1220,7 +1267,7 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
             a.rx_c = cs;
             a.rx_s = sn;
             if (costbw) {
-                if (int rc = get_plan(g, geom, &a.plan))
+                if (int rc = get_plan(g, geom, &a.plan, st->stream))
                     return rc;
                 set_cost(a, -gammas[k], st->init_amp);
                 if (int rc = prepare_phases(st, a, a_log))
@@ -1453,6 +1500,17 @@ struct SlotHost {
     int P = 0, layer = 0;
     int phase = 0;  // 0 free, 1 active, 2 finished (result pending collection)
     std::vector<double> gammas, betas;
+    PlanDev plan{}, eplan{};  // cost and <C> plans of g, fetched at assignment
+    uint64_t gversion = 0;    // g->version the plans belong to
+    uint64_t gepoch = 0;      // g->plan_epoch the plan pointers belong to
+    int has_rr = 0;
+};
+
+// Launch arguments of one slab tile pass (fixed for the slab's lifetime).
+struct SlabPass {
+    TileArgs args;
+    int a_log = 1, slot_shift = 0;
+    bool mirror = false;
 };
 
 struct fq_slab {
@@ -1471,6 +1529,17 @@ struct fq_slab {
     bool pending = false;  // a step was enqueued and not yet observed complete
     double init_amp = 0;
     std::vector<FusedPass> passes;
+    // launch arguments built once at creation
+    TileArgs cost_args{};      // phase tables: pass-1 geometry of the N-qubit state
+    Geom cost_geom{}, exp_geom{};
+    std::vector<SlabPass> pass_args;
+    TileArgs exp_args{};       // <C> over the virtual state
+    int exp_alog = 1, exp_shift = 0;
+    // large slots: each slot's passes are separate tile-pass launches over
+    // its own 2^N (or 2^(N-1)) amplitudes instead of one packed launch
+    bool per_slot = false;
+    std::vector<TileArgs> slot_pass;
+    std::vector<int> slot_pass_alog, slot_pass_z2;
 };
 
 static void slab_free(fq_slab *sb)
@@ -1509,6 +1578,89 @@ static TileArgs slab_cost_args(const fq_slab *sb, Geom *gout)
     if (gout)
         *gout = g1;
     return a1;
+}
+
+// Tile-pass and <C> launch arguments of a slab (its virtual state: the Ns
+// stored qubits, then the slot bits; the mirror pass of a half-state slab
+// keeps qubit N-1 as a position above them).
+static int slab_build_args(fq_slab *sb)
+{
+    const int N = sb->N;
+    const int Ns = sb->z2 ? N - 1 : N;
+    sb->pass_args.clear();
+    for (size_t i = 0; i < sb->passes.size(); ++i) {
+        const FusedPass &ps = sb->passes[i];
+        const int Q = (int)ps.tile.size();
+        SlabPass sp;
+        sp.a_log = tile_a_log(sb->dtype, Q);
+        sp.mirror = sb->z2 && i + 1 == sb->passes.size();
+        std::vector<int> tile = ps.tile;
+        if (sp.mirror)
+            tile.back() = Ns + sb->s;
+        Geom gv = make_geom(Ns + sb->s + (sp.mirror ? 1 : 0), tile, sp.a_log);
+        if (geom_runs(gv) > FQ_MAX_RUNS)
+            return set_err(FQ_ERR_ARG, "internal: slab tile needs too many bit runs");
+        geom_to_args(gv, sp.args);
+        uint32_t gm = 0;
+        for (int q : ps.gates)
+            for (int ti = 0; ti < Q; ++ti)
+                if (ps.tile[ti] == q)
+                    gm |= 1u << ti;
+        if (int rc = set_gates(sp.args, gv, gm))
+            return rc;
+        if (sp.mirror)
+            for (int ti = 0; ti + 1 < Q; ++ti)
+                sp.args.mirror_pm += 1ull << tile[ti];
+        sp.slot_shift = sp.mirror ? N - Q : Ns - Q;
+        sb->pass_args.push_back(sp);
+    }
+    const int Qe = slab_qe(sb);
+    sb->exp_alog = std::max(1, std::min(4, Qe));
+    sb->exp_geom = make_geom(N, low_tile(Qe), sb->exp_alog);
+    geom_to_args(make_geom(Ns + sb->s, low_tile(Qe), sb->exp_alog), sb->exp_args);
+    sb->exp_shift = Ns - Qe;
+    // at least 1024 tiles per slot in the first pass: the persistent
+    // single-state tile kernel per slot (register prefetch, compile-time tile
+    // widths) fills the device on its own
+    const int Q1 = (int)sb->passes[0].tile.size();
+    sb->per_slot = Ns - Q1 >= 10;
+    sb->slot_pass.clear();
+    sb->slot_pass_alog.clear();
+    sb->slot_pass_z2.clear();
+    if (sb->per_slot) {
+        for (size_t i = 0; i < sb->passes.size(); ++i) {
+            const int z2 = !sb->z2 ? Z2_OFF : i + 1 == sb->passes.size() ? Z2_MIRROR : Z2_LOWER;
+            TileArgs a;
+            Geom geom;
+            if (int rc = pass_args(N, sb->dtype, sb->passes[i].tile, sb->passes[i].gates, i == 0, z2, a, geom))
+                return rc;
+            sb->slot_pass.push_back(a);
+            sb->slot_pass_alog.push_back(geom.a_log);
+            sb->slot_pass_z2.push_back(z2);
+        }
+    }
+    return FQ_OK;
+}
+
+// Cost and <C> plans of a slot's graph (again after the graph changed).
+static int slab_slot_plans(fq_slab *sb, SlotHost &h)
+{
+    if (int rc = get_plan(h.g, sb->cost_geom, &h.plan, sb->stream))
+        return rc;
+    if (int rc = get_plan(h.g, sb->exp_geom, &h.eplan, sb->stream))
+        return rc;
+    h.has_rr = 0;
+    for (int j = 0; j < 16; ++j)
+        if (h.plan.rr[j] != 0)
+            h.has_rr = 1;
+    h.gversion = h.g->version;
+    h.gepoch = h.g->plan_epoch;
+    return FQ_OK;
+}
+
+static bool slab_slot_stale(const SlotHost &h)
+{
+    return h.g->version != h.gversion || h.g->plan_epoch != h.gepoch;
 }
 
 extern "C" int fq_slab_create(int N, int nslots, int dtype, fq_slab **out)
@@ -1551,12 +1703,16 @@ extern "C" int fq_slab_create_sched(int N, int nslots, int dtype, int schedule, 
         delete sb;
         return set_err(FQ_ERR_BUDGET, "slab of %d x 2^%d amplitudes exceeds the memory budget", nslots, N);
     }
-    // scratch sizes
-    Geom g1;
-    TileArgs a1 = slab_cost_args(sb, &g1);
-    sb->tab_stride = phase_table_entries(a1, g1.a_log);
+    // launch arguments and scratch sizes
+    sb->cost_args = slab_cost_args(sb, &sb->cost_geom);
+    sb->tab_stride = phase_table_entries(sb->cost_args, sb->cost_geom.a_log);
     const int Qe = slab_qe(sb);
     const size_t etiles = ((size_t)1 << ((z2 ? N - 1 : N) - Qe)) * nslots;
+    if (int rc = slab_build_args(sb)) {
+        budget_give(sb->bytes);
+        delete sb;
+        return rc;
+    }
     bool ok = hipMalloc(&sb->sv, sb->bytes) == hipSuccess;
     if (!ok) {
         hipGetLastError();
@@ -1628,6 +1784,10 @@ extern "C" int fq_slab_assign(fq_slab *sb, int slot, fq_graph *g, int P, const d
     if (h.phase != 0)
         return set_err(FQ_ERR_ARG, "slot %d is not free", slot);
     h.g = g;
+    if (int rc = slab_slot_plans(sb, h)) {
+        h.g = nullptr;
+        return rc;
+    }
     h.P = P;
     h.layer = 0;
     h.phase = 1;
@@ -1667,15 +1827,20 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         FQ_HIP(hipEventSynchronize(sb->done));
         sb->pending = false;
     }
-    const int N = sb->N, K = sb->K;
-    // stored qubits per slot: N, or N-1 for a half-state slab
-    const int Ns = sb->z2 ? N - 1 : N;
-    Geom gl;
-    TileArgs al = slab_cost_args(sb, &gl);
-    const int a1 = gl.a_log;
-    const int Qe = slab_qe(sb);
-    const int ae = std::max(1, std::min(4, Qe));
-    Geom ge = make_geom(N, low_tile(Qe), ae);
+    const int K = sb->K;
+    // refresh plans of changed graphs; a refresh can release a graph's plan
+    // cache, so repeat until every active slot's plans are current
+    for (bool again = true; again;) {
+        again = false;
+        for (int k = 0; k < K; ++k) {
+            SlotHost &h = sb->slots[k];
+            if (h.phase == 1 && slab_slot_stale(h)) {
+                if (int rc = slab_slot_plans(sb, h))
+                    return rc;
+                again = true;
+            }
+        }
+    }
     int nact = 0, nfin = 0;
     for (int k = 0; k < K; ++k) {
         SlotHost &h = sb->slots[k];
@@ -1694,13 +1859,9 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         sp.cost_first = L == 0;
         sp.want_expect = L + 1 == h.P;
         sp.ptab = sb->d_tabs + sb->tab_stride * (size_t)k;
-        if (int rc = get_plan(h.g, gl, &sp.plan))
-            return rc;
-        if (int rc = get_plan(h.g, ge, &sp.eplan))
-            return rc;
-        for (int j = 0; j < 16; ++j)
-            if (sp.plan.rr[j] != 0)
-                sp.has_rr = 1;
+        sp.plan = h.plan;
+        sp.eplan = h.eplan;
+        sp.has_rr = h.has_rr;
         ++nact;
         nfin += sp.want_expect;
     }
@@ -1710,42 +1871,40 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         return FQ_OK;
     FQ_HIP(hipMemcpyAsync(sb->d_params, sb->h_params, sizeof(SlotParams) * K, hipMemcpyHostToDevice,
                           sb->stream));
-    FQ_HIP(launch_slab_phase_table(al, a1, sb->d_params, K, sb->d_tabs, sb->tab_stride, sb->stream));
-    for (size_t i = 0; i < sb->passes.size(); ++i) {
-        const FusedPass &ps = sb->passes[i];
-        const int Q = (int)ps.tile.size();
-        const int a_log = tile_a_log(sb->dtype, Q);
-        // virtual state: the Ns stored qubits, then the slot bits; the mirror
-        // pass of a half-state slab keeps qubit N-1 as a position above them
-        const bool mirror = sb->z2 && i + 1 == sb->passes.size();
-        std::vector<int> tile = ps.tile;
-        if (mirror)
-            tile.back() = Ns + sb->s;
-        Geom gv = make_geom(Ns + sb->s + (mirror ? 1 : 0), tile, a_log);
-        if (geom_runs(gv) > FQ_MAX_RUNS)
-            return set_err(FQ_ERR_ARG, "internal: slab tile needs too many bit runs");
-        TileArgs av;
-        geom_to_args(gv, av);
-        uint32_t gm = 0;
-        for (int q : ps.gates)
-            for (int ti = 0; ti < Q; ++ti)
-                if (ps.tile[ti] == q)
-                    gm |= 1u << ti;
-        if (int rc = set_gates(av, gv, gm))
-            return rc;
-        if (mirror)
-            for (int ti = 0; ti + 1 < Q; ++ti)
-                av.mirror_pm += 1ull << tile[ti];
-        const int slot_shift = mirror ? N - Q : Ns - Q;
-        FQ_HIP(launch_slab_pass(sb->dtype, a_log, i == 0, mirror, sb->sv, av, sb->d_params, slot_shift,
-                                sb->stream));
+    FQ_HIP(launch_slab_phase_table(sb->cost_args, sb->cost_geom.a_log, sb->d_params, K, sb->d_tabs,
+                                   sb->tab_stride, sb->stream));
+    if (sb->per_slot) {
+        for (int k = 0; k < K; ++k) {
+            const SlotParams &sp = sb->h_params[k];
+            if (!sp.active)
+                continue;
+            void *svk = (char *)sb->sv + sb->slot_bytes * (size_t)k;
+            const double2 *ptab = sb->d_tabs + sb->tab_stride * (size_t)k;
+            for (size_t i = 0; i < sb->slot_pass.size(); ++i) {
+                TileArgs a = sb->slot_pass[i];
+                a.rx_c = sp.rx_c;
+                a.rx_s = sp.rx_s;
+                a.gamma = sp.gamma;
+                a.init_amp = sp.init_amp;
+                a.has_rr = sp.has_rr;
+                const int al = sb->slot_pass_alog[i];
+                if (sb->slot_pass_z2[i] >= Z2_MIRROR)
+                    FQ_HIP(launch_tile_mirror(sb->dtype, al, false, svk, a, sb->stream));
+                else
+                    FQ_HIP(launch_tile(sb->dtype, al, i == 0, i == 0 && sp.cost_first, true, svk, a, ptab,
+                                       sb->stream));
+            }
+        }
+    } else {
+        for (size_t i = 0; i < sb->pass_args.size(); ++i) {
+            const SlabPass &sp = sb->pass_args[i];
+            FQ_HIP(launch_slab_pass(sb->dtype, sp.a_log, i == 0, sp.mirror, sb->sv, sp.args, sb->d_params,
+                                    sp.slot_shift, sb->stream));
+        }
     }
     if (nfin > 0) {
-        Geom gev = make_geom(Ns + sb->s, low_tile(Qe), ae);
-        TileArgs aev;
-        geom_to_args(gev, aev);
-        FQ_HIP(launch_slab_expect(sb->dtype, ae, sb->sv, aev, sb->d_params, Ns - Qe, K, sb->d_part, sb->d_res,
-                                  sb->stream));
+        FQ_HIP(launch_slab_expect(sb->dtype, sb->exp_alog, sb->sv, sb->exp_args, sb->d_params, sb->exp_shift, K,
+                                  sb->d_part, sb->d_res, sb->stream));
         FQ_HIP(hipMemcpyAsync(sb->h_res, sb->d_res, sizeof(double) * 2 * K, hipMemcpyDeviceToHost, sb->stream));
     }
     FQ_HIP(hipEventRecord(sb->done, sb->stream));

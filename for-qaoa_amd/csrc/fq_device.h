@@ -1,0 +1,79 @@
+// fq_device.h -- device helpers shared by the tile, adjoint and slab kernels.
+// Internal to libforqaoa.
+#pragma once
+
+#include "fq_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace fq {
+
+template <typename R> struct CT;
+template <> struct CT<double> { typedef double2 T; };
+template <> struct CT<float> { typedef float2 T; };
+
+__device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
+
+// RX(beta) butterfly, the matrix [[c, -i s], [-i s, c]] (src/rx.cu:46-49).
+template <typename T, typename R>
+__device__ __forceinline__ void rx_pair(T &q0, T &q1, R c, R s)
+{
+    T o0, o1;
+    o0.x = q0.x * c + q1.y * s;
+    o0.y = q0.y * c - q1.x * s;
+    o1.x = q0.y * s + q1.x * c;
+    o1.y = -q0.x * s + q1.y * c;
+    q0 = o0;
+    q1 = o1;
+}
+
+template <typename T, typename R, int A_LOG>
+__device__ __forceinline__ void rx_registers(T *v, unsigned mask, R c, R s)
+{
+    constexpr int A = 1 << A_LOG;
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r) {
+        if ((mask >> r) & 1u) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                if (!((j >> r) & 1))
+                    rx_pair<T, R>(v[j], v[j | (1 << r)], c, s);
+        }
+    }
+}
+
+// v *= (c + i s)
+template <typename T, typename R>
+__device__ __forceinline__ void cmul_by(T &v, R c, R s)
+{
+    const R x = v.x, y = v.y;
+    v.x = x * c - y * s;
+    v.y = x * s + y * c;
+}
+
+// Barrier that orders LDS accesses only: outstanding global loads (the next
+// tile's prefetch) stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Scatter the bits of a packed index to physical positions, as runs:
+// index bits [src, src+len) go to positions [dst, dst+len).  Unrolled with a
+// fixed bound so the run descriptors stay in scalar registers.
+__device__ __forceinline__ uint64_t unpack_runs(uint64_t x, const BitRuns &r)
+{
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
+        if (k < r.n) {
+            const uint32_t d = r.run[k];
+            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
+        }
+    }
+    return o;
+}
+
+}  // namespace fq

@@ -101,6 +101,13 @@ struct SwapArgs {
     uint64_t npairs;         // tiled: number of (X <= Y) sub-tile pairs
 };
 
+// Resident work-groups per CU of kernel fn at this block size and dynamic
+// LDS (at least 1), cached per (fn, threads, lds): the runtime query costs
+// tens of microseconds and the persistent launchers ask on every launch.
+int occupancy_per_cu(const void *fn, int threads, size_t lds);
+// Compute units of the current device (cached).
+int device_cu_count();
+
 // Launchers (fq_kernels.hip).  All enqueue on `stream` and return the HIP
 // launch status.
 hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, void *sv,

@@ -22,63 +22,17 @@
 //            through LDS so both sides are read and written coalesced.
 //  k_expect / k_finalize   <C> and sum|psi|^2, FP64, fixed reduction order.
 //  k_topk    per-block top-16 candidates (bitonic sort in LDS).
+#include "fq_device.h"
 #include "fq_internal.h"
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace fq {
-
-template <typename R> struct CT;
-template <> struct CT<double> { typedef double2 T; };
-template <> struct CT<float> { typedef float2 T; };
-
-__device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
-
-// RX(beta) butterfly, the matrix [[c, -i s], [-i s, c]] (src/rx.cu:46-49).
-template <typename T, typename R>
-__device__ __forceinline__ void rx_pair(T &q0, T &q1, R c, R s)
-{
-    T o0, o1;
-    o0.x = q0.x * c + q1.y * s;
-    o0.y = q0.y * c - q1.x * s;
-    o1.x = q0.y * s + q1.x * c;
-    o1.y = -q0.x * s + q1.y * c;
-    q0 = o0;
-    q1 = o1;
-}
-
-template <typename T, typename R, int A_LOG>
-__device__ __forceinline__ void rx_registers(T *v, unsigned mask, R c, R s)
-{
-    constexpr int A = 1 << A_LOG;
-#pragma unroll
-    for (int r = 0; r < A_LOG; ++r) {
-        if ((mask >> r) & 1u) {
-#pragma unroll
-            for (int j = 0; j < A; ++j)
-                if (!((j >> r) & 1))
-                    rx_pair<T, R>(v[j], v[j | (1 << r)], c, s);
-        }
-    }
-}
-
-// v *= (c + i s)
-template <typename T, typename R>
-__device__ __forceinline__ void cmul_by(T &v, R c, R s)
-{
-    const R x = v.x, y = v.y;
-    v.x = x * c - y * s;
-    v.y = x * s + y * c;
-}
-
-// Barrier that orders LDS accesses only: outstanding global loads (the next
-// tile's prefetch) stay in flight across it.
-__device__ __forceinline__ void lds_barrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 // Per-tile part of the cut decomposition, computed by at most two waves:
 // c_i(zH) for every tile bit (lanes i < Q) and K(zH) (one wave, butterfly
@@ -186,22 +140,6 @@ __device__ __forceinline__ void phase_of(double theta, float &c, float &s)
     double r = fma(-n, two_pi_hi, theta);
     r = fma(-n, two_pi_lo, r);
     sincosf((float)r, &s, &c);
-}
-
-// Scatter the bits of a packed index to physical positions, as runs:
-// index bits [src, src+len) go to positions [dst, dst+len).  Unrolled with a
-// fixed bound so the run descriptors stay in scalar registers.
-__device__ __forceinline__ uint64_t unpack_runs(uint64_t x, const BitRuns &r)
-{
-    uint64_t o = 0;
-#pragma unroll
-    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
-        if (k < r.n) {
-            const uint32_t d = r.run[k];
-            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
-        }
-    }
-    return o;
 }
 
 __device__ __forceinline__ uint64_t tile_base(const TileArgs &a, uint64_t tile)
@@ -906,16 +844,50 @@ size_t tile_lds_bytes(int dtype, int Q)
     return n * elem;
 }
 
-static int device_cus()
+int device_cu_count()
 {
-    static int cus = 0;
-    if (cus == 0) {
+    static std::atomic<int> cus{0};
+    int c = cus.load(std::memory_order_relaxed);
+    if (c == 0) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+            c = 256;
+        cus.store(c, std::memory_order_relaxed);
     }
-    return cus;
+    return c;
+}
+
+static int device_cus() { return device_cu_count(); }
+
+int occupancy_per_cu(const void *fn, int threads, size_t lds)
+{
+    struct Key {
+        const void *fn;
+        int threads;
+        size_t lds;
+        bool operator<(const Key &o) const
+        {
+            return std::tie(fn, threads, lds) < std::tie(o.fn, o.threads, o.lds);
+        }
+    };
+    static std::mutex mu;
+    static std::map<Key, int> cache;
+    const Key key{fn, threads, lds};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end())
+            return it->second;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess || per_cu < 1) {
+        hipGetLastError();
+        per_cu = 1;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = per_cu;
+    return per_cu;
 }
 
 template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES>
@@ -937,10 +909,7 @@ static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab
         }
     }
     // persistent grid: as many work-groups as can be resident at once
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
     const uint64_t resident = (uint64_t)per_cu * device_cus();
     // order 3: one tile per work-group (the dispatcher keeps a sliding window)
     const uint64_t grid = tiles < resident || a.order == 3 ? tiles : resident;
@@ -970,10 +939,7 @@ static hipError_t launch_mirror_m(void *sv, const TileArgs &a, hipStream_t strea
             attr_set = true;
         }
     }
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
     const uint64_t resident = (uint64_t)per_cu * device_cus();
     const uint64_t grid = tiles < resident || a.order == 3 ? tiles : resident;
     TileArgs la = a;
@@ -1120,9 +1086,7 @@ static hipError_t launch_swap_pipe(void *sv, const SwapArgs &a, hipStream_t stre
         }
     }
     const uint64_t nunits = a.npairs << a.nr;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess || per_cu < 1)
-        per_cu = 1;
+    const int per_cu = occupancy_per_cu(fn, 256, lds);
     const uint64_t resident = (uint64_t)per_cu * device_cus();
     const uint64_t grid = nunits < resident ? nunits : resident;
     hipLaunchKernelGGL((k_swap_pipe<T, E>), dim3((unsigned)grid), dim3(256), lds, stream,

@@ -8,71 +8,12 @@
 // memory), so requests of different depth P advance together one layer per
 // step and a request can enter a free slot at any layer boundary.
 #include "forqaoa.h"
+#include "fq_device.h"
 #include "fq_internal.h"
 
 #include <hip/hip_runtime.h>
 
 namespace fq {
-
-template <typename R> struct CTS;
-template <> struct CTS<double> { typedef double2 T; };
-template <> struct CTS<float> { typedef float2 T; };
-
-__device__ __forceinline__ int lds_pad_s(int e) { return e + (e >> 4); }
-
-__device__ __forceinline__ void lds_barrier_s()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-__device__ __forceinline__ uint64_t unpack_runs_s(uint64_t x, const BitRuns &r)
-{
-    uint64_t o = 0;
-#pragma unroll
-    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
-        if (k < r.n) {
-            const uint32_t d = r.run[k];
-            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
-        }
-    }
-    return o;
-}
-
-template <typename T, typename R>
-__device__ __forceinline__ void rx_pair_s(T &q0, T &q1, R c, R s)
-{
-    T o0, o1;
-    o0.x = q0.x * c + q1.y * s;
-    o0.y = q0.y * c - q1.x * s;
-    o1.x = q0.y * s + q1.x * c;
-    o1.y = -q0.x * s + q1.y * c;
-    q0 = o0;
-    q1 = o1;
-}
-
-template <typename T, typename R, int A_LOG>
-__device__ __forceinline__ void rx_regs_s(T *v, unsigned mask, R c, R s)
-{
-    constexpr int A = 1 << A_LOG;
-#pragma unroll
-    for (int r = 0; r < A_LOG; ++r)
-        if ((mask >> r) & 1u) {
-#pragma unroll
-            for (int j = 0; j < A; ++j)
-                if (!((j >> r) & 1))
-                    rx_pair_s<T, R>(v[j], v[j | (1 << r)], c, s);
-        }
-}
-
-template <typename T, typename R>
-__device__ __forceinline__ void cmul_s(T &v, R c, R s)
-{
-    const R x = v.x, y = v.y;
-    v.x = x * c - y * s;
-    v.y = x * s + y * c;
-}
 
 // Phase tables of every active slot (layout of k_phase_table per slot, at
 // slot * stride).  Grid: (entries, slots).
@@ -140,12 +81,12 @@ template <typename T, int A, int AH, bool MIRROR>
 __device__ __forceinline__ void slab_load(T *v, const T *__restrict__ sv, const TileArgs &a, uint64_t tl,
                                           uint64_t local_mask, uint64_t toff, const uint64_t *roff)
 {
-    const T *p0 = sv + (unpack_runs_s(tl, a.hruns) + toff);
+    const T *p0 = sv + (unpack_runs(tl, a.hruns) + toff);
 #pragma unroll
     for (int j = 0; j < AH; ++j)
         v[j] = p0[roff[j]];
     if (MIRROR) {
-        const T *pm = sv + (unpack_runs_s(tl ^ local_mask, a.hruns) + a.mirror_pm - toff);
+        const T *pm = sv + (unpack_runs(tl ^ local_mask, a.hruns) + a.mirror_pm - toff);
 #pragma unroll
         for (int j = AH; j < A; ++j)
             v[j] = pm[-(int64_t)roff[j - AH]];
@@ -159,10 +100,10 @@ __device__ __forceinline__ void slab_load(T *v, const T *__restrict__ sv, const 
 // tau ^ local_mask of the same slot at complemented element offsets, so RX on
 // that bit is the butterfly psi(b) <-> psi(~b).
 template <typename R, int A_LOG, bool COST, bool MIRROR>
-__global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restrict__ sv, const TileArgs a,
+__global__ void __launch_bounds__(1024) k_slab_pass(typename CT<R>::T *__restrict__ sv, const TileArgs a,
                                                     const SlotParams *__restrict__ slots, int slot_shift)
 {
-    typedef typename CTS<R>::T T;
+    typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
     constexpr int AH = MIRROR ? A / 2 : A;
     extern __shared__ __align__(16) unsigned char fq_smem_s[];
@@ -170,7 +111,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
     const int t = threadIdx.x;
     const uint64_t ntiles = 1ull << a.nh;
     const uint64_t local_mask = (1ull << slot_shift) - 1ull;
-    const uint64_t toff = unpack_runs_s((uint64_t)t, a.truns);
+    const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     uint64_t roff[A];
 #pragma unroll
     for (int j = 0; j < A; ++j) {
@@ -184,7 +125,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
     const int qmask = (1 << Q) - 1;
     const uint64_t q1 = (uint64_t)Q + 1;
     T *lds = reinterpret_cast<T *>(fq_smem_s);
-    const int nat = lds_pad_s(t);
+    const int nat = lds_pad(t);
 
     // one contiguous range of tiles per work-group; the next live tile's
     // amplitudes load while the current one is in its LDS phases
@@ -202,8 +143,8 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
             slab_load<T, A, AH, MIRROR>(w, sv, a, next, local_mask, toff, roff);
         const SlotParams &sp = slots[tile >> slot_shift];
         const uint64_t lt = tile & local_mask;
-        T *const p0 = sv + (unpack_runs_s(tile, a.hruns) + toff);
-        T *const pm = MIRROR ? sv + (unpack_runs_s(tile ^ local_mask, a.hruns) + a.mirror_pm - toff) : p0;
+        T *const p0 = sv + (unpack_runs(tile, a.hruns) + toff);
+        T *const pm = MIRROR ? sv + (unpack_runs(tile ^ local_mask, a.hruns) + a.mirror_pm - toff) : p0;
         const bool first = COST && sp.cost_first;
         if (COST) {
             const double2 *f_t = sp.ptab + lt * q1;
@@ -236,7 +177,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
                     v[j].x = c0;
                     v[j].y = s0;
                 } else {
-                    cmul_s(v[j], c0, s0);
+                    cmul_by(v[j], c0, s0);
                 }
             }
 #pragma unroll
@@ -248,23 +189,23 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
 #pragma unroll
                 for (int j = 0; j < A; ++j)
                     if ((j >> r) & 1)
-                        cmul_s(v[j], fx, fy);
+                        cmul_by(v[j], fx, fy);
             }
             if (sp.has_rr) {
 #pragma unroll
                 for (int j = 0; j < A; ++j) {
                     const double2 f = rr_tab[j];
-                    cmul_s(v[j], (R)f.x, (R)f.y);
+                    cmul_by(v[j], (R)f.x, (R)f.y);
                 }
             }
         }
         const R c = (R)sp.rx_c, s = (R)sp.rx_s;
-        rx_regs_s<T, R, A_LOG>(v, a.gate_mask >> nthr, c, s);
+        rx_registers<T, R, A_LOG>(v, a.gate_mask >> nthr, c, s);
         if (a.ngroups > 0) {
 #pragma unroll
             for (int j = 0; j < A; ++j)
-                lds[nat + lds_pad_s(j << nthr)] = v[j];
-            lds_barrier_s();
+                lds[nat + lds_pad(j << nthr)] = v[j];
+            lds_barrier();
             for (int g = 0; g < a.ngroups; ++g) {
                 const unsigned gm = a.grp_mask[g];
                 const unsigned rest = (unsigned)qmask & ~gm;
@@ -277,7 +218,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
                             ++k;
                         }
                 }
-                const int etp = lds_pad_s(et);
+                const int etp = lds_pad(et);
                 int pr[A_LOG > 0 ? A_LOG : 1];
                 unsigned g2 = gm;
 #pragma unroll
@@ -293,20 +234,20 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
                     for (int r = 0; r < A_LOG; ++r)
                         if ((j >> r) & 1)
                             e |= 1 << pr[r];
-                    ej[j] = lds_pad_s(e);
+                    ej[j] = lds_pad(e);
                 }
 #pragma unroll
                 for (int j = 0; j < A; ++j)
                     v[j] = lds[etp + ej[j]];
-                rx_regs_s<T, R, A_LOG>(v, a.grp_gates[g], c, s);
+                rx_registers<T, R, A_LOG>(v, a.grp_gates[g], c, s);
 #pragma unroll
                 for (int j = 0; j < A; ++j)
                     lds[etp + ej[j]] = v[j];
-                lds_barrier_s();
+                lds_barrier();
             }
 #pragma unroll
             for (int j = 0; j < A; ++j)
-                v[j] = lds[nat + lds_pad_s(j << nthr)];
+                v[j] = lds[nat + lds_pad(j << nthr)];
         }
 #pragma unroll
         for (int j = 0; j < AH; ++j)
@@ -323,12 +264,12 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CTS<R>::T *__restri
 
 // <C> and sum |psi|^2 of the slots with want_expect; per-tile partials.
 template <typename R, int A_LOG>
-__global__ void __launch_bounds__(256) k_slab_expect(const typename CTS<R>::T *__restrict__ sv,
+__global__ void __launch_bounds__(256) k_slab_expect(const typename CT<R>::T *__restrict__ sv,
                                                      const TileArgs a,
                                                      const SlotParams *__restrict__ slots,
                                                      int slot_shift, double *__restrict__ partials)
 {
-    typedef typename CTS<R>::T T;
+    typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
     __shared__ double s_c[FQ_MAXQ + 2];
     __shared__ double s_K;
@@ -347,7 +288,7 @@ __global__ void __launch_bounds__(256) k_slab_expect(const typename CTS<R>::T *_
     }
     const PlanDev &p = sp.eplan;
     const uint64_t lt = tile & ((1ull << slot_shift) - 1ull);
-    const uint64_t base = unpack_runs_s(tile, a.hruns) + unpack_runs_s((uint64_t)t, a.truns);
+    const uint64_t base = unpack_runs(tile, a.hruns) + unpack_runs((uint64_t)t, a.truns);
     for (int i = t; i < a.Q; i += nthreads) {
         double acc = 0.0;
         for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
@@ -442,18 +383,6 @@ __global__ void __launch_bounds__(256) k_slab_finalize(const double *__restrict_
 
 // ---- launchers -----------------------------------------------------------------
 
-static int cus_s()
-{
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
-    return cus;
-}
-
 hipError_t launch_slab_phase_table(const TileArgs &local, int a_log, const SlotParams *slots, int nslots,
                                    double2 *tabs, size_t stride, hipStream_t stream)
 {
@@ -467,7 +396,7 @@ template <typename R, int A_LOG, bool COST, bool MIRROR>
 static hipError_t launch_slab_pass_t(void *sv, const TileArgs &a, const SlotParams *slots, int slot_shift,
                                      hipStream_t stream)
 {
-    typedef typename CTS<R>::T T;
+    typedef typename CT<R>::T T;
     const size_t lds = a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
     const void *fn = reinterpret_cast<const void *>(&k_slab_pass<R, A_LOG, COST, MIRROR>);
     if (lds > 65536) {
@@ -481,12 +410,9 @@ static hipError_t launch_slab_pass_t(void *sv, const TileArgs &a, const SlotPara
         }
     }
     const unsigned threads = 1u << a.nthr;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
     const uint64_t tiles = 1ull << a.nh;
-    const uint64_t resident = (uint64_t)per_cu * cus_s();
+    const uint64_t resident = (uint64_t)per_cu * device_cu_count();
     const uint64_t grid = tiles < resident ? tiles : resident;
     hipLaunchKernelGGL((k_slab_pass<R, A_LOG, COST, MIRROR>), dim3((unsigned)grid), dim3(threads), lds, stream,
                        reinterpret_cast<T *>(sv), a, slots, slot_shift);
@@ -527,7 +453,7 @@ hipError_t launch_slab_expect(int dtype, int a_log, const void *sv, const TileAr
     const dim3 g((unsigned)tiles), b(1u << a.nthr);
 #define FQ_SE(R, AL)                                                                                  \
     hipLaunchKernelGGL((k_slab_expect<R, AL>), g, b, 0, stream,                                      \
-                       reinterpret_cast<const typename CTS<R>::T *>(sv), a, slots, slot_shift, partials); \
+                       reinterpret_cast<const typename CT<R>::T *>(sv), a, slots, slot_shift, partials); \
     break;
     if (dtype == 0) {
         switch (a_log) {

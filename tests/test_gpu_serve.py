@@ -31,7 +31,7 @@ def solo(N, edges, g, b, dtype):
 
 @pytest.mark.parametrize("schedule", ["fused", "z2"])
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
-@pytest.mark.parametrize("N", [15, 21])
+@pytest.mark.parametrize("N", [15, 21, 25])  # 25: each slot runs its own tile-pass launches
 def test_slab_slots_match_solo_runs(dtype, schedule, N):
     sb = Slab(N, 8, dtype, schedule)
     jobs = []
