@@ -736,73 +736,120 @@ __device__ __forceinline__ bool better(double pa, uint64_t ia, double pb, uint64
     return pa > pb || (pa == pb && ia < ib);
 }
 
+// Wave-wide best (p, i) by butterfly; every lane receives it and the lane
+// it came from.
+__device__ __forceinline__ void wave_best(double &p, uint64_t &i, int &from)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double op = __shfl_xor(p, off, 64);
+        const uint64_t oi = ((uint64_t)(uint32_t)__shfl_xor((int)(uint32_t)i, off, 64)) |
+                            ((uint64_t)(uint32_t)__shfl_xor((int)(i >> 32), off, 64) << 32);
+        const int of = __shfl_xor(from, off, 64);
+        if (better(op, oi, p, i)) {
+            p = op;
+            i = oi;
+            from = of;
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src)
+{
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)x, src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(x >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d)
+{
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)x, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(x >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Per-block top-16 (probability, index) candidates.  Each wave keeps its 16
+// best in lanes 0..15 (one entry per lane, descending).  Amplitudes stream 8
+// per thread per step (8 loads in flight); an amplitude is a candidate only if
+// it beats the wave's 16th (one compare and a ballot), and candidates are
+// inserted one at a time with lane shuffles.  One wave then merges the 4
+// waves' 64 entries into the block's 16.
 template <typename T>
 __global__ void __launch_bounds__(256) k_topk(const T *__restrict__ sv, uint64_t n,
                                               double *__restrict__ cand_p,
                                               uint64_t *__restrict__ cand_i)
 {
     constexpr int K = FQ_TOPK_MAX;
-    __shared__ double sp[256 * K];
-    __shared__ uint64_t si[256 * K];
-    double lp[K];
-    uint64_t li[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        lp[k] = -1.0;
-        li[k] = ~0ull;
-    }
+    constexpr int U = 8;
+    __shared__ double s_p[4 * K];
+    __shared__ uint64_t s_i[4 * K];
     const int t = threadIdx.x;
-    for (uint64_t b = (uint64_t)blockIdx.x * 256 + t; b < n; b += (uint64_t)gridDim.x * 256) {
-        const T v = sv[b];
-        const double re = (double)v.x, im = (double)v.y;
-        const double p = re * re + im * im;
-        if (better(p, b, lp[K - 1], li[K - 1])) {
-            lp[K - 1] = p;
-            li[K - 1] = b;
+    const int lane = t & 63, wave = t >> 6;
+    double lp = -1.0;       // this lane's list entry (lanes < K)
+    uint64_t li = ~0ull;
+    double thr_p = -1.0;    // the wave's K-th best
+    uint64_t thr_i = ~0ull;
+    const uint64_t step = (uint64_t)gridDim.x * 256 * U;
+    for (uint64_t base = (uint64_t)blockIdx.x * 256 * U + t; base < n; base += step) {
+        T v[U];
 #pragma unroll
-            for (int k = K - 1; k > 0; --k) {
-                if (better(lp[k], li[k], lp[k - 1], li[k - 1])) {
-                    const double tp = lp[k];
-                    lp[k] = lp[k - 1];
-                    lp[k - 1] = tp;
-                    const uint64_t ti = li[k];
-                    li[k] = li[k - 1];
-                    li[k - 1] = ti;
+        for (int u = 0; u < U; ++u) {
+            const uint64_t b = base + (uint64_t)u * 256;
+            if (b < n)
+                v[u] = sv[b];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t b = base + (uint64_t)u * 256;
+            const double re = (double)v[u].x, im = (double)v[u].y;
+            const double pb = re * re + im * im;
+            uint64_t mask = __ballot(b < n && better(pb, b, thr_p, thr_i));
+            while (mask) {
+                const int src = __ffsll((unsigned long long)mask) - 1;
+                mask &= mask - 1;
+                const double cp = __shfl(pb, src, 64);
+                const uint64_t ci = shfl_u64(b, src);
+                const uint64_t gt = __ballot(lane < K && better(cp, ci, lp, li));
+                if (gt) {
+                    const int pos = __ffsll((unsigned long long)gt) - 1;
+                    const double up = __shfl_up(lp, 1, 64);
+                    const uint64_t ui = shfl_up_u64(li, 1);
+                    if (lane > pos && lane < K) {
+                        lp = up;
+                        li = ui;
+                    }
+                    if (lane == pos) {
+                        lp = cp;
+                        li = ci;
+                    }
+                    thr_p = __shfl(lp, K - 1, 64);
+                    thr_i = shfl_u64(li, K - 1);
                 }
             }
         }
     }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        sp[t * K + k] = lp[k];
-        si[t * K + k] = li[k];
+    if (lane < K) {
+        s_p[wave * K + lane] = lp;
+        s_i[wave * K + lane] = li;
     }
     __syncthreads();
-    constexpr int M = 256 * K;
-    for (int kk = 2; kk <= M; kk <<= 1) {
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            for (int i = t; i < M; i += 256) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const bool asc = (i & kk) == 0;
-                    const bool sw = asc ? better(sp[ixj], si[ixj], sp[i], si[i])
-                                        : better(sp[i], si[i], sp[ixj], si[ixj]);
-                    if (sw) {
-                        const double tp = sp[i];
-                        sp[i] = sp[ixj];
-                        sp[ixj] = tp;
-                        const uint64_t ti = si[i];
-                        si[i] = si[ixj];
-                        si[ixj] = ti;
-                    }
-                }
+    if (wave == 0) {
+        double cp = s_p[lane];
+        uint64_t ci = s_i[lane];
+        for (int r = 0; r < K; ++r) {
+            double bp = cp;
+            uint64_t bi = ci;
+            int from = lane;
+            wave_best(bp, bi, from);
+            if (lane == 0) {
+                cand_p[(uint64_t)blockIdx.x * K + r] = bp;
+                cand_i[(uint64_t)blockIdx.x * K + r] = bi;
             }
-            __syncthreads();
+            if (lane == from) {
+                cp = -2.0;
+                ci = ~0ull;
+            }
         }
-    }
-    if (t < K) {
-        cand_p[(uint64_t)blockIdx.x * K + t] = sp[t];
-        cand_i[(uint64_t)blockIdx.x * K + t] = si[t];
     }
 }
 

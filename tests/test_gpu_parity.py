@@ -359,3 +359,37 @@ def test_n30_z2_closed_form_and_fused_agree(dtype):
         assert d_l2 < TOL[dtype]
     finally:
         st.destroy()
+
+
+# ---- top-k over crafted states -----------------------------------------------
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("N,kind", [(5, "random"), (13, "random"), (22, "random"), (18, "flat"),
+                                    (20, "spiky"), (12, "pairs")])
+def test_topk_matches_numpy(dtype, N, kind):
+    rng = np.random.default_rng(N)
+    n = 1 << N
+    if kind == "random":
+        psi = rng.normal(size=n) + 1j * rng.normal(size=n)
+    elif kind == "flat":
+        psi = np.ones(n, dtype=np.complex128)
+    elif kind == "spiky":
+        psi = 1e-3 * (rng.normal(size=n) + 1j * rng.normal(size=n))
+        psi[rng.choice(n, 40, replace=False)] = np.linspace(1.0, 2.0, 40)
+    else:  # equal pairs far apart: ties broken by the smaller index
+        psi = 1e-3 * rng.random(n).astype(np.complex128)
+        top = rng.choice(n // 2, 10, replace=False)
+        psi[top] = psi[n - 1 - top] = np.linspace(1.0, 2.0, 10)
+    psi /= np.linalg.norm(psi)
+    st = State(N, C=min(N, 10), dtype=dtype)
+    try:
+        st.copy_in(psi)
+        amp = st.copy_out().astype(np.complex128)
+        p = np.abs(amp) ** 2
+        order = np.lexsort((np.arange(n), -p))
+        for k in (16, 1):
+            idx, prob = st.topk(k)
+            np.testing.assert_array_equal(idx.astype(np.int64), order[:k])
+            np.testing.assert_allclose(prob, p[order[:k]], rtol=1e-12)
+    finally:
+        st.destroy()
