@@ -108,7 +108,17 @@ def cpu_baseline(f=3):
     t1 = time.perf_counter()
     del sv
     full = (t1 - t0) * (1 << f)
-    return dict(value=1.0 / full, unit="layers/s", cores=O.num_threads(), kind="port",
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=1.0 / full, unit="layers/s", cores=O.num_threads(), kind="port", cpu_model=model,
+                host_cpus=os.cpu_count(),
                 sample=(f"oracle (C restatement of src/Weighted.cpp:115-137, complex128, OpenMP) on one "
                         f"layer of N=30 G(30,0.5) P=5 C=13, first 1/{1 << f} of each operation's work units "
                         f"({t1 - t0:.2f} s), scaled x{1 << f}: {full:.1f} s per full layer"))
