@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
         }
         lds_barrier();
         for (int g = 0; g < a.ngroups; ++g) {
-            const unsigned gm = a.grp_mask[g];
+            const unsigned gm = group_mask(a, g);
             const unsigned rest = (unsigned)qmask & ~gm;
             int et = 0;
             {
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 vp[j] = ldsP[etp + ej[j]];
                 vl[j] = ldsL[etp + ej[j]];
             }
-            rx_both<T, R, A_LOG>(vp, vl, a.grp_gates[g], c, s, accX);
+            rx_both<T, R, A_LOG>(vp, vl, group_gates(a, g), c, s, accX);
 #pragma unroll
             for (int j = 0; j < A; ++j) {
                 ldsP[etp + ej[j]] = vp[j];

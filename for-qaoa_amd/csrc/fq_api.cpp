@@ -298,6 +298,8 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
         if ((gate_mask >> i) & 1u)
             pend.push_back(i);
     a.ngroups = 0;
+    a.grp_mask_w = 0;
+    a.grp_gates_w = 0;
     size_t pos = 0;
     while (pos < pend.size()) {
         if (a.ngroups >= FQ_MAX_GROUPS)
@@ -322,6 +324,9 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
         }
         a.grp_mask[a.ngroups] = gm;
         a.grp_gates[a.ngroups] = gg;
+        if (a.ngroups < 4)
+            a.grp_mask_w |= (uint64_t)gm << (16 * a.ngroups);
+        a.grp_gates_w |= (uint32_t)(gg & 0xf) << (4 * a.ngroups);
         a.ngroups++;
     }
     return FQ_OK;

@@ -76,4 +76,20 @@ __device__ __forceinline__ uint64_t unpack_runs(uint64_t x, const BitRuns &r)
     return o;
 }
 
+// Tile-bit mask and register gate bits of LDS phase g.  Read from packed
+// words with a shift: an indexed kernel-argument array would be loaded from
+// memory inside the group loop, and the wait for that load (the counters are
+// in issue order) would also wait for the next tile's prefetch and the
+// previous tile's stores.
+__device__ __forceinline__ unsigned group_mask(const TileArgs &a, int g)
+{
+    static_assert(FQ_MAX_GROUPS == 5, "four packed masks plus one");
+    return g < 4 ? (unsigned)((a.grp_mask_w >> (16 * g)) & 0xffffu) : (unsigned)a.grp_mask[4];
+}
+
+__device__ __forceinline__ unsigned group_gates(const TileArgs &a, int g)
+{
+    return (a.grp_gates_w >> (4 * g)) & 0xfu;
+}
+
 }  // namespace fq

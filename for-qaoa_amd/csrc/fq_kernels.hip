@@ -160,6 +160,15 @@ __device__ __forceinline__ double2 lane_bcast(double2 v, int lane)
                         __builtin_bit_cast(double, ((unsigned long long)yh << 32) | yl));
 }
 
+// v, once it has arrived: the select needs the loaded value, so the wait for
+// it is issued here and not inside a later loop (where the in-order wait
+// counters would also wait for that loop's stores).  The factors are finite
+// phase factors, so the select always returns v.
+__device__ __forceinline__ double2 landed(double2 v)
+{
+    return __builtin_isfinite(v.x) ? v : make_double2(1.0, 0.0);
+}
+
 // deposit the low bits of x into the set bits of mask (q-bit masks)
 template <int QT>
 __device__ __forceinline__ int deposit(int x, unsigned mask, int q)
@@ -255,6 +264,21 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     double2 fcur = make_double2(0.0, 0.0), fnext = make_double2(0.0, 0.0);
     if (ROWPF && tile < ntiles)
         fcur = ptab[tile * q1 + row_lane];
+    // the thread's own factors do not depend on the tile: loaded once, so no
+    // load issued after a tile's stores has to wait for them
+    double2 f_thr = make_double2(1.0, 0.0);
+    double2 f_reg[A_LOG > 0 ? A_LOG : 1];
+    // compile-time widths have at least 64 threads: lane j holds
+    // register-register factor j (broadcast with readlane)
+    double2 f_rr = make_double2(1.0, 0.0);
+    if (COST) {
+        f_thr = landed(thr_tab[t]);
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            f_reg[r] = landed(thr_tab[((1 + r) << nthr) + t]);
+        if (ROWPF)
+            f_rr = landed(rr_tab[(t & 63) % A]);
+    }
     T v[A];
     if (!FIRST && tile < ntiles) {
         const T *p0 = sv + (tile_base(a, tile) + toff);
@@ -278,7 +302,6 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         if (COST) {
             const double2 *f_t = ptab + tile * q1;
             auto row = [&](int i) -> double2 { return ROWPF ? lane_bcast(fcur, i) : f_t[i]; };
-            const double2 f_thr = thr_tab[t];
             double x = f_thr.x, y = f_thr.y;
             zmul(x, y, row(Q));
 #pragma unroll
@@ -308,8 +331,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             }
 #pragma unroll
             for (int r = 0; r < A_LOG; ++r) {
-                const double2 fr = thr_tab[((1 + r) << nthr) + t];
-                double fx = fr.x, fy = fr.y;
+                double fx = f_reg[r].x, fy = f_reg[r].y;
                 zmul(fx, fy, row(nthr + r));
                 const R cf = (R)fx, sf = (R)fy;
 #pragma unroll
@@ -320,7 +342,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             if (a.has_rr) {
 #pragma unroll
                 for (int j = 0; j < A; ++j) {
-                    const double2 f = rr_tab[j];
+                    const double2 f = ROWPF ? lane_bcast(f_rr, j) : rr_tab[j];
                     cmul_by(v[j], (R)f.x, (R)f.y);
                 }
             }
@@ -358,7 +380,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
 #pragma unroll 1
                 for (int g = 0; g < a.ngroups; ++g) {
                     const int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
-                    const unsigned gm = a.grp_mask[g];
+                    const unsigned gm = group_mask(a, g);
                     int pr[A_LOG > 0 ? A_LOG : 1];
                     unsigned g2 = gm;
 #pragma unroll
@@ -379,7 +401,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
 #pragma unroll
                     for (int j = 0; j < A; ++j)
                         v[j] = lds[eg + ej[j]];
-                    rx_registers<T, R, A_LOG>(v, a.grp_gates[g], c, s);
+                    rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
 #pragma unroll
                     for (int j = 0; j < A; ++j)
                         lds[eg + ej[j]] = v[j];

@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CT<R>::T *__restric
                 lds[nat + lds_pad(j << nthr)] = v[j];
             lds_barrier();
             for (int g = 0; g < a.ngroups; ++g) {
-                const unsigned gm = a.grp_mask[g];
+                const unsigned gm = group_mask(a, g);
                 const unsigned rest = (unsigned)qmask & ~gm;
                 int et = 0;
                 {
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CT<R>::T *__restric
 #pragma unroll
                 for (int j = 0; j < A; ++j)
                     v[j] = lds[etp + ej[j]];
-                rx_registers<T, R, A_LOG>(v, a.grp_gates[g], c, s);
+                rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
 #pragma unroll
                 for (int j = 0; j < A; ++j)
                     lds[etp + ej[j]] = v[j];
