@@ -1,7 +1,7 @@
 // weighted -- QAOA Max-Cut driver with the reference's command line:
 //
 //   weighted <P> <N> <D> <C> <B> [--graph FILE] [--angles FILE] [--dtype c128|c64]
-//            [--schedule reference|fused] [--dump FILE] [--report]
+//            [--schedule reference|fused|z2] [--dump FILE] [--report]
 //
 // Positionals and defaults (1 21 0 10 19) as src/Weighted.cpp:52-71.  Without
 // --graph the instance is the reference's own: the complete graph K_N with unit
@@ -88,9 +88,9 @@ int main(int argc, char **argv)
         fprintf(stderr, "weighted: --dtype must be c128 or c64\n");
         return 2;
     }
-    const int sched = sched_s == "fused" ? FQ_SCHED_FUSED : FQ_SCHED_REFERENCE;
-    if (sched_s != "fused" && sched_s != "reference") {
-        fprintf(stderr, "weighted: --schedule must be reference or fused\n");
+    const int sched = sched_s == "fused" ? FQ_SCHED_FUSED : sched_s == "z2" ? FQ_SCHED_Z2 : FQ_SCHED_REFERENCE;
+    if (sched_s != "fused" && sched_s != "reference" && sched_s != "z2") {
+        fprintf(stderr, "weighted: --schedule must be reference, fused or z2\n");
         return 2;
     }
 
