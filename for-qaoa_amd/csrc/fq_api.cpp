@@ -157,18 +157,27 @@ static hipEvent_t pool_get(fq_state *st)
 
 static int timing_collect(fq_state *st)
 {
-    for (auto &r : st->pending) {
-        FQ_HIP(hipEventSynchronize(r.e1));
+    size_t done = 0;
+    int rc = FQ_OK;
+    for (; done < st->pending.size(); ++done) {
+        const TimingRec &r = st->pending[done];
+        hipError_t e = hipEventSynchronize(r.e1);
         float ms = 0;
-        FQ_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+        if (e == hipSuccess)
+            e = hipEventElapsedTime(&ms, r.e0, r.e1);
+        if (e != hipSuccess) {
+            rc = set_err(FQ_ERR_DEVICE, "timing event failed: %s", hipGetErrorString(e));
+            break;
+        }
         st->t_ms[r.kind] += ms;
         st->t_cnt[r.kind] += 1;
         st->t_bytes[r.kind] += r.bytes;
         st->pool.push_back(r.e0);
         st->pool.push_back(r.e1);
     }
-    st->pending.clear();
-    return FQ_OK;
+    // collected records leave the list (a failed one stays for a later call)
+    st->pending.erase(st->pending.begin(), st->pending.begin() + (std::ptrdiff_t)done);
+    return rc;
 }
 
 // Wrap one launch with events when timing is on.
@@ -184,17 +193,26 @@ static int timed(fq_state *st, int kind, double bytes, F &&launch)
         }
         e0 = pool_get(st);
         e1 = pool_get(st);
-        if (!e0 || !e1)
-            return set_err(FQ_ERR_DEVICE, "hipEventCreate failed");
-        FQ_HIP(hipEventRecord(e0, st->stream));
+        if (!e0 || !e1 || hipEventRecord(e0, st->stream) != hipSuccess) {
+            if (e0)
+                st->pool.push_back(e0);
+            if (e1)
+                st->pool.push_back(e1);
+            return set_err(FQ_ERR_DEVICE, "timing event unavailable");
+        }
     }
     hipError_t e = launch();
-    if (e != hipSuccess)
+    if (e == hipSuccess && st->timing)
+        e = hipEventRecord(e1, st->stream);
+    if (e != hipSuccess) {
+        if (st->timing) {
+            st->pool.push_back(e0);
+            st->pool.push_back(e1);
+        }
         return set_err(FQ_ERR_DEVICE, "kernel launch failed: %s", hipGetErrorString(e));
-    if (st->timing) {
-        FQ_HIP(hipEventRecord(e1, st->stream));
-        st->pending.push_back({kind, e0, e1, bytes});
     }
+    if (st->timing)
+        st->pending.push_back({kind, e0, e1, bytes});
     return FQ_OK;
 }
 
