@@ -22,14 +22,17 @@
 
 namespace fq {
 
-// Im(conj(l0) p1 + conj(l1) p0): the pair's contribution to Im<lambda|X_j|psi>
-template <typename T>
-__device__ __forceinline__ double im_x_pair(const T &l0, const T &l1, const T &p0, const T &p1)
+// Im(conj(l0) p1 + conj(l1) p0): the pair's contribution to Im<lambda|X_j|psi>,
+// in the amplitude precision (products of two stored amplitudes)
+template <typename T, typename R>
+__device__ __forceinline__ R im_x_pair(const T &l0, const T &l1, const T &p0, const T &p1)
 {
-    return ((double)l0.x * (double)p1.y - (double)l0.y * (double)p1.x) +
-           ((double)l1.x * (double)p0.y - (double)l1.y * (double)p0.x);
+    return (l0.x * p1.y - l0.y * p1.x) + (l1.x * p0.y - l1.y * p0.x);
 }
 
+// RX on the register gate bits of both states; the A/2 pair terms of each
+// gate bit are summed in the amplitude precision and added to the FP64
+// accumulator once per bit.
 template <typename T, typename R, int A_LOG>
 __device__ __forceinline__ void rx_both(T *vp, T *vl, unsigned mask, R c, R s, double &acc)
 {
@@ -37,15 +40,17 @@ __device__ __forceinline__ void rx_both(T *vp, T *vl, unsigned mask, R c, R s, d
 #pragma unroll
     for (int r = 0; r < A_LOG; ++r) {
         if ((mask >> r) & 1u) {
+            R part = 0;
 #pragma unroll
             for (int j = 0; j < A; ++j) {
                 if (!((j >> r) & 1)) {
                     const int j1 = j | (1 << r);
-                    acc += im_x_pair(vl[j], vl[j1], vp[j], vp[j1]);
+                    part += im_x_pair<T, R>(vl[j], vl[j1], vp[j], vp[j1]);
                     rx_pair<T, R>(vp[j], vp[j1], c, s);
                     rx_pair<T, R>(vl[j], vl[j1], c, s);
                 }
             }
+            acc += (double)part;
         }
     }
 }
@@ -54,20 +59,24 @@ __device__ __forceinline__ void rx_both(T *vp, T *vl, unsigned mask, R c, R s, d
 // One uncompute pass over a tile geometry for psi and lambda together:
 // RX(rx angle) on the gate bits (accumulating Im<lambda|X_j|psi>), then, if
 // COSTBW, Im<lambda|S|phi> from the per-amplitude cut value and the diagonal
-// phase from the phase table (built for -gamma) on both.  partials[2*block]
-// and [2*block+1] receive the work-group's two sums.
+// phase on both.  Phase factors (built for -gamma) and cut coefficients come
+// from the two tables of k_phase_table, same layout: s/2 of an amplitude is
+// the sum of its tile's K entry, its tile coefficients c_i for the set thread
+// and register bits, the thread's entry, its register entries and the
+// register-register entry, and its phase is the product of the same entries
+// of ptab.  partials[2*block] and [2*block+1] receive the work-group's two
+// sums.
 template <typename R, int A_LOG, bool COSTBW>
 __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict__ psi,
                                                     typename CT<R>::T *__restrict__ lam,
                                                     const TileArgs a,
                                                     const double2 *__restrict__ ptab,
+                                                    const double *__restrict__ ctab,
                                                     double *__restrict__ partials)
 {
     typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
     extern __shared__ __align__(16) unsigned char fq_smem_a[];
-    __shared__ double s_c[FQ_MAXQ + 2];
-    __shared__ double s_K;
     __shared__ double s_red[2][8];
 
     const int Q = a.Q, nthr = a.nthr;
@@ -91,8 +100,35 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     const R c = (R)a.rx_c, s = (R)a.rx_s;
     const int qmask = (1 << Q) - 1;
     const uint64_t q1 = (uint64_t)Q + 1;
-    const double2 *thr_tab = COSTBW ? ptab + ntiles * q1 : nullptr;
-    const double2 *rr_tab = COSTBW ? thr_tab + ((size_t)(1 + A_LOG) << nthr) : nullptr;
+    const uint64_t thr0 = ntiles * q1;                          // per-thread entries
+    const uint64_t rr0 = thr0 + ((uint64_t)(1 + A_LOG) << nthr);  // register-register
+    const int lane = t & 63;
+    // A tile's Q+1 table entries sit in lanes 0..Q and are broadcast with
+    // readlane when the wave has that many lanes (and A for the register-
+    // register entries); otherwise they are read from the tables.
+    const bool rowpf = (1 << nthr) >= (Q + 1 > A ? Q + 1 : A);
+    // tile-independent entries of this thread, loaded once
+    double c_thr = 0.0, c_rr = 0.0;
+    double2 f_thr = make_double2(1.0, 0.0), f_rr = make_double2(1.0, 0.0);
+    double c_reg[A_LOG > 0 ? A_LOG : 1];
+    double2 f_reg[A_LOG > 0 ? A_LOG : 1];
+    if (COSTBW) {
+        c_thr = landed(ctab[thr0 + t]);
+        f_thr = landed(ptab[thr0 + t]);
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r) {
+            c_reg[r] = landed(ctab[thr0 + ((uint64_t)(1 + r) << nthr) + t]);
+            f_reg[r] = landed(ptab[thr0 + ((uint64_t)(1 + r) << nthr) + t]);
+        }
+        if (rowpf) {
+            c_rr = landed(ctab[rr0 + lane % A]);
+            f_rr = landed(ptab[rr0 + lane % A]);
+        }
+    }
+    // the tile's row of both tables (lane i: entry i), and the next tile's
+    double crow = 0.0, cnext = 0.0;
+    double2 frow = make_double2(1.0, 0.0), fnext = make_double2(1.0, 0.0);
+    const bool row_lane = rowpf && lane <= Q;
 
     double accX = 0.0, accG = 0.0;
     // one contiguous range of tiles per work-group; the next tile's
@@ -110,6 +146,10 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
             vp[j] = psi[b0 + roff[j]];
             vl[j] = lam[b0 + roff[j]];
         }
+        if (COSTBW && row_lane) {
+            crow = ctab[first_tile * q1 + lane];
+            frow = ptab[first_tile * q1 + lane];
+        }
     }
     for (uint64_t tile = first_tile; tile < end_tile; ++tile) {
         const uint64_t base = unpack_runs(tile, a.hruns) + toff;
@@ -120,6 +160,10 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 vp[j] = psi[base + roff[j]];
                 vl[j] = lam[base + roff[j]];
             }
+            if (COSTBW && row_lane) {
+                crow = ctab[tile * q1 + lane];
+                frow = ptab[tile * q1 + lane];
+            }
         }
         if (PREFETCH && tile + 1 < end_tile) {
             const uint64_t b1 = unpack_runs(tile + 1, a.hruns) + toff;
@@ -128,23 +172,9 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 wp[j] = psi[b1 + roff[j]];
                 wl[j] = lam[b1 + roff[j]];
             }
-        }
-        if (COSTBW) {
-            // per-tile cut coefficients (published by the first LDS barrier)
-            for (int i = t; i < Q; i += nthreads) {
-                double acc = 0.0;
-                for (int e = a.plan.csr_off[i]; e < a.plan.csr_off[i + 1]; ++e)
-                    acc += a.plan.csr_w[e] * (double)((tile >> a.plan.csr_h[e]) & 1ull);
-                s_c[i] = a.plan.cH[i] - 2.0 * acc;
-            }
-            if (t == 0) {
-                double K = 0.0;
-                for (int e = 0; e < a.plan.n_hh; ++e)
-                    K += a.plan.hh_w[e] *
-                         (double)(((tile >> a.plan.hh_u[e]) ^ (tile >> a.plan.hh_v[e])) & 1ull);
-                for (int h = 0; h < a.nh; ++h)
-                    K += a.plan.omega[h] * (double)((tile >> h) & 1ull);
-                s_K = K;
+            if (COSTBW && row_lane) {
+                cnext = ctab[(tile + 1) * q1 + lane];
+                fnext = ptab[(tile + 1) * q1 + lane];
             }
         }
         rx_both<T, R, A_LOG>(vp, vl, a.gate_mask >> nthr, c, s, accX);
@@ -159,17 +189,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
         lds_barrier();
         for (int g = 0; g < a.ngroups; ++g) {
             const unsigned gm = group_mask(a, g);
-            const unsigned rest = (unsigned)qmask & ~gm;
-            int et = 0;
-            {
-                int k = 0;
-                for (int i = 0; i < Q; ++i)
-                    if ((rest >> i) & 1u) {
-                        et |= ((t >> k) & 1) << i;
-                        ++k;
-                    }
-            }
-            const int etp = lds_pad(et);
+            const int etp = lds_pad(deposit_runs(t, (unsigned)qmask & ~gm));
             int pr[A_LOG > 0 ? A_LOG : 1];
             unsigned g2 = gm;
 #pragma unroll
@@ -208,62 +228,65 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
             }
         }
         if (COSTBW) {
-            // cut value of each register: base + register coefficients + rr
-            double cbase = s_K + a.plan.thr[t];
-            for (int i = 0; i < nthr; ++i)
-                cbase += s_c[i] * (double)((t >> i) & 1);
-            double cr[A_LOG > 0 ? A_LOG : 1];
-#pragma unroll
-            for (int r = 0; r < A_LOG; ++r)
-                cr[r] = s_c[nthr + r] + a.plan.thr[((1 + r) << nthr) + t];
-            // phase factors exp(-i gamma s/2) from the table
+            const double *c_t = ctab + tile * q1;
             const double2 *f_t = ptab + tile * q1;
-            const double2 f_thr = thr_tab[t];
+            auto crow_at = [&](int i) -> double { return rowpf ? lane_bcast(crow, i) : c_t[i]; };
+            auto frow_at = [&](int i) -> double2 { return rowpf ? lane_bcast(frow, i) : f_t[i]; };
+            // s/2 and the phase factor of register 0: tile K entry, the
+            // thread's entry and the tile coefficients of the set thread bits
+            double half = crow_at(Q) + c_thr;
             double x = f_thr.x, y = f_thr.y;
             {
-                const double2 f = f_t[Q];
+                const double2 f = frow_at(Q);
                 const double nx = x * f.x - y * f.y;
                 y = x * f.y + y * f.x;
                 x = nx;
             }
             for (int i = 0; i < nthr; ++i) {
-                const double2 f = f_t[i];
+                const double ci = crow_at(i);
+                const double2 f = frow_at(i);
+                const bool on = (t >> i) & 1;
                 const double nx = x * f.x - y * f.y;
                 const double ny = x * f.y + y * f.x;
-                const bool on = (t >> i) & 1;
+                half += on ? ci : 0.0;
                 x = on ? nx : x;
                 y = on ? ny : y;
             }
+            double cr[A_LOG > 0 ? A_LOG : 1];
+            double2 fr[A_LOG > 0 ? A_LOG : 1];
+#pragma unroll
+            for (int r = 0; r < A_LOG; ++r) {
+                cr[r] = crow_at(nthr + r) + c_reg[r];
+                const double2 fq = frow_at(nthr + r);
+                fr[r] = make_double2(f_reg[r].x * fq.x - f_reg[r].y * fq.y,
+                                     f_reg[r].x * fq.y + f_reg[r].y * fq.x);
+            }
 #pragma unroll
             for (int j = 0; j < A; ++j) {
-                double cut = cbase + a.plan.rr[j];
+                double h = half + (rowpf ? lane_bcast(c_rr, j) : ctab[rr0 + j]);
 #pragma unroll
                 for (int r = 0; r < A_LOG; ++r)
                     if ((j >> r) & 1)
-                        cut += cr[r];
-                const double sj = 2.0 * cut - a.plan.wtot;
-                // Im(conj(l) * s * p)
-                accG += sj * ((double)vl[j].x * (double)vp[j].y - (double)vl[j].y * (double)vp[j].x);
+                        h += cr[r];
+                // s = 2 Cut - W = 2 h;  Im(conj(l) * s * p)
+                const R im = vl[j].x * vp[j].y - vl[j].y * vp[j].x;
+                accG += 2.0 * h * (double)im;
                 cmul_by(vp[j], (R)x, (R)y);
                 cmul_by(vl[j], (R)x, (R)y);
             }
 #pragma unroll
             for (int r = 0; r < A_LOG; ++r) {
-                const double2 fr = thr_tab[((1 + r) << nthr) + t];
-                const double2 fq = f_t[nthr + r];
-                const double fx = fr.x * fq.x - fr.y * fq.y;
-                const double fy = fr.x * fq.y + fr.y * fq.x;
 #pragma unroll
                 for (int j = 0; j < A; ++j)
                     if ((j >> r) & 1) {
-                        cmul_by(vp[j], (R)fx, (R)fy);
-                        cmul_by(vl[j], (R)fx, (R)fy);
+                        cmul_by(vp[j], (R)fr[r].x, (R)fr[r].y);
+                        cmul_by(vl[j], (R)fr[r].x, (R)fr[r].y);
                     }
             }
             if (a.has_rr) {
 #pragma unroll
                 for (int j = 0; j < A; ++j) {
-                    const double2 f = rr_tab[j];
+                    const double2 f = rowpf ? lane_bcast(f_rr, j) : ptab[rr0 + j];
                     cmul_by(vp[j], (R)f.x, (R)f.y);
                     cmul_by(vl[j], (R)f.x, (R)f.y);
                 }
@@ -274,9 +297,8 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
             psi[base + roff[j]] = vp[j];
             lam[base + roff[j]] = vl[j];
         }
-        // the next tile's first LDS writes reuse this thread's own slots; the
-        // cut coefficients are rewritten only after every thread passed the
-        // barrier below
+        // the next tile's LDS phases start after every thread has read its
+        // amplitudes of this one back
         lds_barrier();
         if (PREFETCH) {
 #pragma unroll
@@ -284,6 +306,8 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 vp[j] = wp[j];
                 vl[j] = wl[j];
             }
+            crow = cnext;
+            frow = fnext;
         }
     }
     // fixed-order work-group reduction: waves by shuffle, then 16 slots
@@ -292,7 +316,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
         accX += __shfl_xor(accX, off, 64);
         accG += __shfl_xor(accG, off, 64);
     }
-    const int wave = t >> 6, lane = t & 63;
+    const int wave = t >> 6;
     if (lane == 0) {
         s_red[0][wave] = accX;
         s_red[1][wave] = accG;
@@ -396,7 +420,8 @@ size_t pair_lds_bytes(int dtype, int Q)
 // pair each, at most the tile count).
 template <typename R, int A_LOG, bool COSTBW>
 static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const double2 *ptab,
-                                double *partials, unsigned *grid_out, hipStream_t stream)
+                                const double *ctab, double *partials, unsigned *grid_out,
+                                hipStream_t stream)
 {
     typedef typename CT<R>::T T;
     const size_t lds = pair_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q);
@@ -418,16 +443,17 @@ static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const d
     const unsigned grid = (unsigned)(tiles < resident ? tiles : resident);
     *grid_out = grid;
     hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW>), dim3(grid), dim3(threads), lds, stream,
-                       reinterpret_cast<T *>(psi), reinterpret_cast<T *>(lam), a, ptab, partials);
+                       reinterpret_cast<T *>(psi), reinterpret_cast<T *>(lam), a, ptab, ctab, partials);
     return hipGetLastError();
 }
 
 hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
-                       const double2 *ptab, double *partials, unsigned *grid, hipStream_t stream)
+                       const double2 *ptab, const double *ctab, double *partials, unsigned *grid,
+                       hipStream_t stream)
 {
-#define FQ_PAIR(R, AL)                                                                             \
-    return costbw ? launch_pair_t<R, AL, true>(psi, lam, a, ptab, partials, grid, stream)         \
-                  : launch_pair_t<R, AL, false>(psi, lam, a, ptab, partials, grid, stream);
+#define FQ_PAIR(R, AL)                                                                              \
+    return costbw ? launch_pair_t<R, AL, true>(psi, lam, a, ptab, ctab, partials, grid, stream)    \
+                  : launch_pair_t<R, AL, false>(psi, lam, a, ptab, ctab, partials, grid, stream);
     if (dtype == 0) {
         switch (a_log) {
             case 1: FQ_PAIR(double, 1)

@@ -128,7 +128,7 @@ struct fq_state {
     uint64_t *d_cand_i = nullptr;
     int cand_blocks = 0;
     double2 *d_phase = nullptr;  // per-launch cost phase factors
-    size_t phase_cap = 0;        // entries
+    size_t phase_cap = 0;        // bytes
     void *aux = nullptr;         // adjoint state (lambda), allocated on first use
     double *d_grad = nullptr;    // adjoint per-pass sums
     size_t grad_cap = 0;
@@ -807,19 +807,25 @@ static void set_cost(TileArgs &a, double gamma, double init_amp)
 
 // Phase factors for a cost-phase tile launch (grows the state's table; a
 // larger table needs the previous launches that read the old one to finish).
-static int prepare_phases(fq_state *st, TileArgs &a, int a_log)
+// Phase table of one cost launch into d_phase; with cut, the table of cut
+// coefficients follows it in the same buffer (*cut points there).
+static int prepare_phases(fq_state *st, TileArgs &a, int a_log, double **cut = nullptr)
 {
-    const size_t need = phase_table_entries(a, a_log);
+    const size_t n = phase_table_entries(a, a_log);
+    const size_t need = n * sizeof(double2) + (cut ? n * sizeof(double) : 0);
     if (need > st->phase_cap) {
         FQ_HIP(hipStreamSynchronize(st->stream));
         hipFree(st->d_phase);
         st->d_phase = nullptr;
         st->phase_cap = 0;
-        if (hipMalloc(&st->d_phase, need * sizeof(double2)) != hipSuccess)
-            return set_err(FQ_ERR_OOM, "phase table allocation of %zu bytes failed", need * sizeof(double2));
+        if (hipMalloc(&st->d_phase, need) != hipSuccess)
+            return set_err(FQ_ERR_OOM, "phase table allocation of %zu bytes failed", need);
         st->phase_cap = need;
     }
-    FQ_HIP(launch_phase_table(a, a_log, st->d_phase, st->stream));
+    double *c = cut ? reinterpret_cast<double *>(st->d_phase + n) : nullptr;
+    FQ_HIP(launch_phase_table(a, a_log, st->d_phase, c, st->stream));
+    if (cut)
+        *cut = c;
     return FQ_OK;
 }
 
@@ -1289,18 +1295,20 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
             sincos(-betas[k] / 2, &sn, &cs);
             a.rx_c = cs;
             a.rx_s = sn;
+            double *ctab = nullptr;
             if (costbw) {
                 if (int rc = get_plan(g, geom, &a.plan, st->stream))
                     return rc;
                 set_cost(a, -gammas[k], st->init_amp);
-                if (int rc = prepare_phases(st, a, a_log))
+                if (int rc = prepare_phases(st, a, a_log, &ctab))
                     return rc;
             }
             double *slot = st->d_grad + 2 * (1 + (size_t)k * npass + i);
             int rc = timed(st, FQ_K_ADJOINT, 4.0 * (double)st->bytes, [&] {
                 unsigned grid = 0;
                 hipError_t e = launch_pair(st->dtype, a_log, costbw, st->sv, st->aux, a,
-                                           costbw ? st->d_phase : nullptr, st->d_part, &grid, st->stream);
+                                           costbw ? st->d_phase : nullptr, ctab, st->d_part, &grid,
+                                           st->stream);
                 if (e != hipSuccess)
                     return e;
                 return launch_finalize(st->d_part, grid, slot, st->stream);

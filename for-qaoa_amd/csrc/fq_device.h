@@ -51,6 +51,50 @@ __device__ __forceinline__ void cmul_by(T &v, R c, R s)
     v.y = x * s + y * c;
 }
 
+// Value of v in lane `lane` of the wave (lane uniform).
+__device__ __forceinline__ double lane_bcast(double v, int lane)
+{
+    const long long i = __builtin_bit_cast(long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)i, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(i >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double2 lane_bcast(double2 v, int lane)
+{
+    return make_double2(lane_bcast(v.x, lane), lane_bcast(v.y, lane));
+}
+
+// v, once it has arrived: the select needs the loaded value, so the wait for
+// it is issued here and not inside a later loop (where the in-order wait
+// counters would also wait for that loop's stores).  Callers pass finite
+// values (phase factors, cut coefficients), so the select always returns v.
+__device__ __forceinline__ double2 landed(double2 v)
+{
+    return __builtin_isfinite(v.x) ? v : make_double2(1.0, 0.0);
+}
+
+__device__ __forceinline__ double landed(double v)
+{
+    return __builtin_isfinite(v) ? v : 0.0;
+}
+
+// Deposit the low bits of x into the set bits of a uniform mask, one run of
+// set bits at a time (a mask with A_LOG bits cleared has at most A_LOG+1 runs).
+__device__ __forceinline__ int deposit_runs(int x, unsigned mask)
+{
+    int e = 0;
+    while (mask) {
+        const int pos = __builtin_ctz(mask);
+        const int len = __builtin_ctz(~(mask >> pos));
+        const unsigned run = len >= 32 ? 0xffffffffu : (1u << len) - 1u;
+        e |= (int)(((unsigned)x & run) << pos);
+        x = len >= 32 ? 0 : (int)((unsigned)x >> len);
+        mask &= ~(run << pos);
+    }
+    return e;
+}
+
 // Barrier that orders LDS accesses only: outstanding global loads (the next
 // tile's prefetch) stay in flight across it.
 __device__ __forceinline__ void lds_barrier()

@@ -120,8 +120,10 @@ hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, 
 // half of the full state.
 hipError_t launch_tile_mirror(int dtype, int a_log, bool expand, void *sv, const TileArgs &a,
                               hipStream_t stream);
-// Phase factors of a cost-phase launch into tab (size phase_table_entries).
-hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, hipStream_t stream);
+// Phase factors of a cost-phase launch into tab (size phase_table_entries);
+// with cut not null, their exponents over gamma into cut (same size).
+hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, double *cut,
+                              hipStream_t stream);
 size_t phase_table_entries(const TileArgs &a, int a_log);
 hipError_t launch_swap(int dtype, void *sv, const SwapArgs &a, hipStream_t stream);
 hipError_t launch_expect(int dtype, int a_log, const void *sv, const TileArgs &a, double *partials,
@@ -134,8 +136,11 @@ hipError_t launch_fill_zero_one(int dtype, void *sv, int N, hipStream_t stream);
 hipError_t launch_finalize(const double *partials, uint64_t n, double *out2, hipStream_t stream);
 // adjoint (fq_adjoint.hip)
 // grid receives the number of work-groups (= partial sum pairs written).
+// With costbw, ptab and ctab are the phase and cut tables of one
+// launch_phase_table call.
 hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
-                       const double2 *ptab, double *partials, unsigned *grid, hipStream_t stream);
+                       const double2 *ptab, const double *ctab, double *partials, unsigned *grid,
+                       hipStream_t stream);
 hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, const TileArgs &a,
                           double *partials, hipStream_t stream);
 size_t pair_lds_bytes(int dtype, int Q);

@@ -81,16 +81,19 @@ __device__ __forceinline__ void cut_tile_prologue(const TileArgs &a, uint64_t ti
 //   tab[ntiles*(Q+1) + ((1+r) << nthr) + t] = exp(i*gamma*d_r(t))
 // and finally the 16 register-register factors exp(i*gamma*rr[j]),
 // so that exp(i*gamma*s/2) = exp(i*gamma*(Cut - W/2)) of every amplitude is a
-// product of table entries.
+// product of table entries.  If cut is not null it receives the exponents
+// divided by gamma at the same indices (c_i, K - W/2, thr, d_r, rr): their
+// sums give s/2 = Cut - W/2 of every amplitude.
 __global__ void __launch_bounds__(256) k_phase_table(const TileArgs a, int a_log,
-                                                     double2 *__restrict__ tab)
+                                                     double2 *__restrict__ tab,
+                                                     double *__restrict__ cut)
 {
     const PlanDev &p = a.plan;
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t q1 = (uint64_t)a.Q + 1;
     const uint64_t ntile_e = run_tiles(a) * q1;
     const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
-    double angle;
+    double coef;
     if (gid < ntile_e) {
         const uint64_t tile = gid / q1;
         const int i = (int)(gid - tile * q1);
@@ -98,25 +101,27 @@ __global__ void __launch_bounds__(256) k_phase_table(const TileArgs a, int a_log
             double acc = 0.0;
             for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
                 acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
-            angle = a.gamma * (p.cH[i] - 2.0 * acc);
+            coef = p.cH[i] - 2.0 * acc;
         } else {
             double K = 0.0;
             for (int e = 0; e < p.n_hh; ++e)
                 K += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
             for (int h = 0; h < a.nh; ++h)
                 K += p.omega[h] * (double)((tile >> h) & 1ull);
-            angle = a.gamma * (K - 0.5 * p.wtot);
+            coef = K - 0.5 * p.wtot;
         }
     } else if (gid < ntile_e + nthr_e) {
-        angle = a.gamma * p.thr[gid - ntile_e];
+        coef = p.thr[gid - ntile_e];
     } else if (gid < ntile_e + nthr_e + 16) {
-        angle = a.gamma * p.rr[gid - ntile_e - nthr_e];
+        coef = p.rr[gid - ntile_e - nthr_e];
     } else {
         return;
     }
     double sn, cs;
-    sincos(angle, &sn, &cs);
+    sincos(a.gamma * coef, &sn, &cs);
     tab[gid] = make_double2(cs, sn);
+    if (cut)
+        cut[gid] = coef;
 }
 
 __device__ __forceinline__ void zmul(double &x, double &y, double2 f)
@@ -145,28 +150,6 @@ __device__ __forceinline__ void phase_of(double theta, float &c, float &s)
 __device__ __forceinline__ uint64_t tile_base(const TileArgs &a, uint64_t tile)
 {
     return unpack_runs(tile, a.hruns);
-}
-
-// Value of v in lane `lane` of the wave (lane uniform).
-__device__ __forceinline__ double2 lane_bcast(double2 v, int lane)
-{
-    const long long xi = __builtin_bit_cast(long long, v.x);
-    const long long yi = __builtin_bit_cast(long long, v.y);
-    const unsigned xl = (unsigned)__builtin_amdgcn_readlane((int)xi, lane);
-    const unsigned xh = (unsigned)__builtin_amdgcn_readlane((int)(xi >> 32), lane);
-    const unsigned yl = (unsigned)__builtin_amdgcn_readlane((int)yi, lane);
-    const unsigned yh = (unsigned)__builtin_amdgcn_readlane((int)(yi >> 32), lane);
-    return make_double2(__builtin_bit_cast(double, ((unsigned long long)xh << 32) | xl),
-                        __builtin_bit_cast(double, ((unsigned long long)yh << 32) | yl));
-}
-
-// v, once it has arrived: the select needs the loaded value, so the wait for
-// it is issued here and not inside a later loop (where the in-order wait
-// counters would also wait for that loop's stores).  The factors are finite
-// phase factors, so the select always returns v.
-__device__ __forceinline__ double2 landed(double2 v)
-{
-    return __builtin_isfinite(v.x) ? v : make_double2(1.0, 0.0);
 }
 
 // deposit the low bits of x into the set bits of mask (q-bit masks)
@@ -1130,11 +1113,13 @@ size_t phase_table_entries(const TileArgs &a, int a_log)
     return (size_t)run_tiles(a) * (size_t)(a.Q + 1) + ((size_t)(1 + a_log) << a.nthr) + 16;
 }
 
-hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, hipStream_t stream)
+hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, double *cut,
+                              hipStream_t stream)
 {
     const size_t n = phase_table_entries(a, a_log);
     const size_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_phase_table, dim3((unsigned)blocks), dim3(256), 0, stream, a, a_log, tab);
+    hipLaunchKernelGGL(k_phase_table, dim3((unsigned)blocks), dim3(256), 0, stream, a, a_log, tab,
+                       cut);
     return hipGetLastError();
 }
 

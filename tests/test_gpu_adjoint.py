@@ -42,7 +42,7 @@ def central_fd(f, g, b, h):
     return gg, gb
 
 
-@pytest.mark.parametrize("N,P,seed", [(6, 2, 1), (10, 3, 2), (13, 2, 3), (16, 3, 4)])
+@pytest.mark.parametrize("N,P,seed", [(4, 2, 5), (5, 3, 6), (6, 2, 1), (10, 3, 2), (13, 2, 3), (16, 3, 4)])
 @pytest.mark.parametrize("dtype,tol", [("c128", 1e-7), ("c64", 1e-4)])
 def test_adjoint_vs_oracle_central_difference(N, P, seed, dtype, tol):
     edges = G.gnp(N, 0.5, 900 + seed, "uniform")
