@@ -938,13 +938,35 @@ static int pass_tile_q(int dtype)
     return env_q > 0 && env_q < mx ? env_q : mx;
 }
 
-// Tile passes covering every qubit once: a contiguous low tile of Qmax
-// qubits, then the rest spread evenly over strided tiles that carry the
-// lowest qubits (at least 128 contiguous bytes per run).
+// One tile pass of a fused layer: its tile positions and RX targets.
 struct FusedPass {
     std::vector<int> tile, gates;
 };
 
+// Qubits per block when the strided passes' new qubits are dealt out (see
+// cover_passes): 2 by default; FQ_PASS_BLOCK (environment) selects another,
+// 0 for contiguous blocks.
+static int pass_block()
+{
+    static int blk = -1;
+    if (blk < 0) {
+        const char *e = getenv("FQ_PASS_BLOCK");
+        blk = e ? std::max(0, atoi(e)) : 2;
+    }
+    return blk;
+}
+
+// The tile passes of one fused layer, covering every qubit once.  Pass 0 is
+// the low Qmax qubits (cost phase and their RX); the other qubits go to the
+// fewest passes that fit, each tile being the lowest qubits as carriers
+// (runs of at least 128 contiguous bytes) plus the pass's new qubits.  The new qubits are dealt to the
+// passes in blocks, from the top qubit down and from the last pass down, so
+// the last pass holds qubit N-1 (the half-state mirror pass needs it) and
+// every strided pass mixes middle and high index bits.  On MI355X this
+// spreads a pass's accesses over the DRAM better than one contiguous block
+// of qubits per pass (N=30 complex128: strided passes 8.0 and 7.2 ms -> 6.8
+// and 7.0 ms; N=34 complex64: 65 and 98 ms -> 57 and 81 ms).  A pass that
+// would need more bit runs than the tile kernel takes gets larger blocks.
 static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
 {
     std::vector<FusedPass> ps;
@@ -959,21 +981,37 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
     const int cmin = std::min(carriers_for(dtype), Qmax - 1);
     const int per_max = Qmax - cmin;
     const int npass = (rest + per_max - 1) / per_max;
-    int next = Qmax;
-    for (int k = 0; k < npass; ++k) {
-        const int n_new = rest / npass + (k < rest % npass ? 1 : 0);
-        const int c = std::min(Qmax - n_new, next);
-        FusedPass p;
-        for (int i = 0; i < c; ++i)
-            p.tile.push_back(i);
-        for (int i = 0; i < n_new; ++i) {
-            p.tile.push_back(next + i);
-            p.gates.push_back(next + i);
+    std::vector<int> want(npass);
+    for (int k = 0; k < npass; ++k)
+        want[k] = rest / npass + (k < rest % npass ? 1 : 0);
+    for (int blk = pass_block() > 0 ? pass_block() : rest;; ++blk) {
+        std::vector<std::vector<int>> news(npass);
+        int q = N - 1, k = npass - 1;
+        while (q >= Qmax) {
+            for (int i = 0; i < blk && q >= Qmax && (int)news[k].size() < want[k]; ++i)
+                news[k].push_back(q--);
+            k = (k + npass - 1) % npass;
         }
-        next += n_new;
-        ps.push_back(p);
+        std::vector<FusedPass> more;
+        bool fits = true;
+        for (int k2 = 0; k2 < npass; ++k2) {
+            std::sort(news[k2].begin(), news[k2].end());
+            FusedPass p;
+            for (int i = 0; i < Qmax - want[k2]; ++i)
+                p.tile.push_back(i);
+            for (int nq : news[k2]) {
+                p.tile.push_back(nq);
+                p.gates.push_back(nq);
+            }
+            const int Q = (int)p.tile.size();
+            fits = fits && geom_runs(make_geom(N, p.tile, tile_a_log(dtype, Q))) <= FQ_MAX_RUNS;
+            more.push_back(p);
+        }
+        if (fits || blk >= rest) {
+            ps.insert(ps.end(), more.begin(), more.end());
+            return ps;
+        }
     }
-    return ps;
 }
 
 // Split arbitrary RX targets into tile passes (targets below C: one chunk pass).
@@ -1507,6 +1545,66 @@ extern "C" int fq_timing_get(fq_state *st, int kind, double *ms, int64_t *count,
         *count = st->t_cnt[kind];
     if (bytes)
         *bytes = st->t_bytes[kind];
+    return FQ_OK;
+}
+
+extern "C" int fq_stream_probe(size_t bytes, int reps, double *inplace_gbs, double *read_gbs,
+                               double *write_gbs)
+{
+    if (bytes < ((size_t)1 << 20) || bytes % 8 || reps < 1 || !inplace_gbs || !read_gbs || !write_gbs)
+        return set_err(FQ_ERR_ARG, "stream probe needs bytes >= 1 MiB (a multiple of 8), reps >= 1 and outputs");
+    if (!budget_take(bytes))
+        return set_err(FQ_ERR_BUDGET, "stream probe of %zu bytes exceeds the memory budget", bytes);
+    void *buf = nullptr;
+    float *sink = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double out[3] = {0, 0, 0};
+    int rc = FQ_OK;
+    hipError_t e = hipMalloc(&buf, bytes);
+    if (e == hipSuccess)
+        e = hipMalloc(&sink, sizeof(float));
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&e0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&e1);
+    if (e == hipSuccess)
+        e = launch_stream(buf, bytes, 2, sink, s);  // first touch
+    // modes 0 (in place), 1 (read), 2 (fill): traffic 2, 1, 1 x bytes
+    for (int mode = 0; mode < 3 && e == hipSuccess; ++mode) {
+        e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; ++r)
+            e = launch_stream(buf, bytes, mode, sink, s);
+        if (e == hipSuccess)
+            e = hipEventRecord(e1, s);
+        if (e == hipSuccess)
+            e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess)
+            e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms > 0.0f)
+            out[mode] = (mode == 0 ? 2.0 : 1.0) * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e != hipSuccess)
+        rc = set_err(FQ_ERR_DEVICE, "stream probe: %s", hipGetErrorString(e));
+    if (s)
+        hipStreamSynchronize(s);
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    if (s)
+        hipStreamDestroy(s);
+    hipFree(sink);
+    hipFree(buf);
+    budget_give(bytes);
+    if (rc)
+        return rc;
+    *inplace_gbs = out[0];
+    *read_gbs = out[1];
+    *write_gbs = out[2];
     return FQ_OK;
 }
 
