@@ -81,7 +81,7 @@ def measure(name, dtype, schedule, steps, warmup, C=None, dominant=None):
     res = dict(workload=name, N=N, P=P, C=C, dtype=dtype, schedule=schedule, steps=steps,
                seconds=secs, layers_per_s=P * steps / secs, ms_per_step=secs / steps * 1e3,
                layer_hbm_frac=(total_bytes / secs) / (HBM_PEAK_GBS * 1e9),
-               state_gb=st.nbytes / 1e9, expect_cut=ex, norm2=n2,
+               state_gb=st.nbytes / 1e9, state_bytes=st.nbytes, expect_cut=ex, norm2=n2,
                top8=[[int(i), float(p)] for i, p in zip(idx, prob)], kernels=kern)
     st.destroy()
     g.destroy()
@@ -266,6 +266,16 @@ def main():
 
     main_res = measure(args.workload, args.dtype, args.schedule, args.steps, args.warmup)
     log(f"main: {main_res['layers_per_s']:.2f} layers/s, layer HBM frac {main_res['layer_hbm_frac']:.3f}")
+    # what simple streaming kernels reach over a buffer of the state's size
+    stream = None
+    try:
+        from forqaoa import stream_probe
+
+        stream = stream_probe(main_res["state_bytes"], 3)
+        log(f"stream probe over {stream['bytes'] / 1e9:.2f} GB: in-place {stream['inplace_gbs']:.0f}, "
+            f"read {stream['read_gbs']:.0f}, write {stream['write_gbs']:.0f} GB/s")
+    except Exception as e:
+        stream = dict(error=repr(e))
     extra = []
     if not args.no_extra:
         for name, dt, sch, steps, warm in [("n30_reg3", "c128", "fused", 3, 1),
@@ -313,6 +323,11 @@ def main():
                 kernel=("k_tile (fused tile pass: cost phase + RX through LDS)" if dom_name == "fused"
                         else "k_tile (cost phase)"),
                 bytes_per_launch=dom.get("bytes_per_launch"), avg_launch_ms=dom.get("avg_ms"))
+    if stream and stream.get("inplace_gbs"):
+        # the dominant kernel reads and writes the state: compare with the
+        # in-place streaming rate measured on the same device
+        roof["stream_inplace_gbs"] = stream["inplace_gbs"]
+        roof["frac_of_stream_inplace"] = (dom.get("gbs") or 0) / stream["inplace_gbs"]
     out = {
         "metric": "QAOA layers/s at N=30,34 + %HBM peak; served evals/s with p50/p99 latency",
         "value": main_res["layers_per_s"],
@@ -338,6 +353,7 @@ def main():
         "extra": extra,
         "adjoint": adjoint,
         "serving": serving,
+        "stream_probe": stream,
     }
     print(json.dumps(out), flush=True)
 

@@ -133,6 +133,9 @@ hipError_t launch_topk(int dtype, const void *sv, int N, int nblocks, double *ca
 hipError_t launch_gather(int dtype, const void *sv, const uint64_t *idx, size_t n, double *out,
                          hipStream_t stream);
 hipError_t launch_fill_zero_one(int dtype, void *sv, int N, hipStream_t stream);
+// streaming probe over bytes (a multiple of 8): mode 0 in-place scale,
+// 1 read-only, 2 fill; sink: one float
+hipError_t launch_stream(void *p, size_t bytes, int mode, float *sink, hipStream_t stream);
 hipError_t launch_finalize(const double *partials, uint64_t n, double *out2, hipStream_t stream);
 // adjoint (fq_adjoint.hip)
 // grid receives the number of work-groups (= partial sum pairs written).

@@ -1108,6 +1108,49 @@ hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, 
     return launch_tile_r<float>(a_log, cost, first, gates, sv, a, ptab, stream);
 }
 
+// Streaming probe (fq_stream_probe): mode 0 scales n 8-byte elements in
+// place, mode 1 reads them (sum kept live through sink), mode 2 fills them.
+// One work-group per block of 1024 x 16 elements (128 KiB), no persistence:
+// the form that streamed fastest among the shapes tried on MI355X.
+__global__ void __launch_bounds__(1024) k_stream(float2 *__restrict__ p, uint64_t n, int mode, float scale,
+                                                 float *__restrict__ sink)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * (1024ull * 16) + threadIdx.x;
+    float2 v[16];
+    if (mode != 2) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (base + 1024ull * j < n)
+                v[j] = p[base + 1024ull * j];
+    }
+    if (mode == 1) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (base + 1024ull * j < n)
+                acc += v[j].x + v[j].y;
+        if (acc == 1.0f)
+            sink[0] = acc;
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (base + 1024ull * j < n)
+            p[base + 1024ull * j] = mode == 2 ? make_float2(scale, scale)
+                                              : make_float2(v[j].x * scale, v[j].y * scale);
+}
+
+hipError_t launch_stream(void *p, size_t bytes, int mode, float *sink, hipStream_t stream)
+{
+    const uint64_t n = bytes / 8;
+    const uint64_t blocks = (n + 1024 * 16 - 1) / (1024 * 16);
+    if (blocks > 0xffffffffull)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stream, dim3((unsigned)blocks), dim3(1024), 0, stream, reinterpret_cast<float2 *>(p), n,
+                       mode, mode == 2 ? 0.0f : 1.0f, sink);
+    return hipGetLastError();
+}
+
 size_t phase_table_entries(const TileArgs &a, int a_log)
 {
     return (size_t)run_tiles(a) * (size_t)(a.Q + 1) + ((size_t)(1 + a_log) << a.nthr) + 16;

@@ -254,6 +254,15 @@ class State:
             pass
 
 
+def stream_probe(nbytes, reps=3):
+    """In-place, read-only and write-only streaming rates (GB/s of HBM traffic)
+    of simple kernels over an nbytes buffer: what a pass that streams a state
+    of that size could reach on this device."""
+    ip, rd, wr = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib().fq_stream_probe(int(nbytes), int(reps), ctypes.byref(ip), ctypes.byref(rd), ctypes.byref(wr)))
+    return dict(inplace_gbs=ip.value, read_gbs=rd.value, write_gbs=wr.value, bytes=int(nbytes))
+
+
 def init_amp(N):
     """(1/sqrt 2)^N as initH computes it (src/rzz.cu:26-28)."""
     return math.pow(1.0 / math.sqrt(2.0), N)

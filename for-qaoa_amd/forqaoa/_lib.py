@@ -99,6 +99,7 @@ SIGNATURES = {
     "fq_timing_enable": (_i, [_vp, _i]),
     "fq_timing_get": (_i, [_vp, _i, _dp, _i64p, _dp]),
     "fq_timing_reset": (_i, [_vp]),
+    "fq_stream_probe": (_i, [_sz, _i, _dp, _dp, _dp]),
 }
 
 

@@ -214,6 +214,15 @@ int fq_timing_enable(fq_state *st, int on);
 int fq_timing_get(fq_state *st, int kind, double *ms, int64_t *count, double *bytes);
 int fq_timing_reset(fq_state *st);
 
+/* ---- device streaming rates (no counterpart in the reference: a yardstick
+ * for the passes above, which all stream the state through HBM) */
+
+/* Allocates `bytes` (>= 1 MiB, a multiple of 8; counted against the memory
+ * budget), then times `reps` launches each of an in-place scale (read and
+ * write every byte), a read-only sum and a fill over it, and frees it.
+ * Rates in GB/s of HBM traffic (2 x bytes per in-place launch). */
+int fq_stream_probe(size_t bytes, int reps, double *inplace_gbs, double *read_gbs, double *write_gbs);
+
 #ifdef __cplusplus
 }
 #endif

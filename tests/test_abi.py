@@ -130,3 +130,17 @@ def test_server_reports_requests_that_never_fit():
     finally:
         srv.close()
         _lib.check(_lib.lib().fq_set_memory_budget(0))
+
+
+def test_stream_probe_validates_before_allocating():
+    L = _lib.lib()
+    d = ctypes.c_double()
+    assert L.fq_stream_probe(1 << 10, 1, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == _lib.FQ_ERR_ARG
+    assert L.fq_stream_probe((1 << 20) + 4, 1, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == _lib.FQ_ERR_ARG
+    assert L.fq_stream_probe(1 << 20, 0, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == _lib.FQ_ERR_ARG
+    assert L.fq_stream_probe(1 << 20, 1, None, None, None) == _lib.FQ_ERR_ARG
+    _lib.check(L.fq_set_memory_budget(1 << 20))
+    try:
+        assert L.fq_stream_probe(2 << 20, 1, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == _lib.FQ_ERR_BUDGET
+    finally:
+        _lib.check(L.fq_set_memory_budget(0))

@@ -393,3 +393,13 @@ def test_topk_matches_numpy(dtype, N, kind):
             np.testing.assert_allclose(prob, p[order[:k]], rtol=1e-12)
     finally:
         st.destroy()
+
+
+def test_stream_probe_rates_are_plausible():
+    # yardstick kernels over 256 MiB: finite, positive rates below 3x the
+    # 8 TB/s HBM peak (the buffer partly fits the last-level cache)
+    from forqaoa import stream_probe
+    r = stream_probe(256 << 20, 2)
+    print(f"stream probe 256 MiB: {r}", flush=True)
+    for k in ("inplace_gbs", "read_gbs", "write_gbs"):
+        assert 0.0 < r[k] < 24000.0
