@@ -995,16 +995,36 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
         std::vector<FusedPass> more;
         bool fits = true;
         for (int k2 = 0; k2 < npass; ++k2) {
-            std::sort(news[k2].begin(), news[k2].end());
+            std::vector<int> &nw = news[k2];
+            std::sort(nw.begin(), nw.end());
+            const int c = Qmax - want[k2];
+            const int a_log = tile_a_log(dtype, Qmax);
+            // lane order: the 64 lanes of a wave are the tile's first six
+            // positions; past the carriers they take the pass's lowest new
+            // qubits whose byte-address bit is outside [19, 22] (one load
+            // instruction whose lanes differ only in those bits measured up
+            // to 1.6x slower), else the highest; never N-1, which stays the
+            // top register bit for the half-state mirror pass
+            const int lanes = std::max(0, std::min(6, Qmax - a_log) - c);
+            const int elem_log = dtype == FQ_C128 ? 4 : 3;
+            std::vector<int> lane_q;
+            for (int q : nw)
+                if ((int)lane_q.size() < lanes && q != N - 1 && (q + elem_log < 19 || q + elem_log > 22))
+                    lane_q.push_back(q);
+            for (int i = (int)nw.size() - 1; i >= 0 && (int)lane_q.size() < lanes; --i)
+                if (nw[i] != N - 1 && std::find(lane_q.begin(), lane_q.end(), nw[i]) == lane_q.end())
+                    lane_q.push_back(nw[i]);
+            std::sort(lane_q.begin(), lane_q.end());
             FusedPass p;
-            for (int i = 0; i < Qmax - want[k2]; ++i)
+            for (int i = 0; i < c; ++i)
                 p.tile.push_back(i);
-            for (int nq : news[k2]) {
-                p.tile.push_back(nq);
-                p.gates.push_back(nq);
-            }
-            const int Q = (int)p.tile.size();
-            fits = fits && geom_runs(make_geom(N, p.tile, tile_a_log(dtype, Q))) <= FQ_MAX_RUNS;
+            for (int q : lane_q)
+                p.tile.push_back(q);
+            for (int q : nw)
+                if (std::find(lane_q.begin(), lane_q.end(), q) == lane_q.end())
+                    p.tile.push_back(q);
+            p.gates = nw;
+            fits = fits && geom_runs(make_geom(N, p.tile, tile_a_log(dtype, Qmax))) <= FQ_MAX_RUNS;
             more.push_back(p);
         }
         if (fits || blk >= rest) {
