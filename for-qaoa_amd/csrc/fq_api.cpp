@@ -956,6 +956,19 @@ static int pass_block()
     return blk;
 }
 
+// Whether the carried low qubits' RX moves out of the cost pass (see
+// cover_passes): on by default; FQ_CARRIER_GATES=0 (environment) keeps every
+// low qubit's RX in pass 0.
+static bool carrier_gates()
+{
+    static int on = -1;
+    if (on < 0) {
+        const char *e = getenv("FQ_CARRIER_GATES");
+        on = e ? (atoi(e) != 0) : 1;
+    }
+    return on != 0;
+}
+
 // The tile passes of one fused layer, covering every qubit once.  Pass 0 is
 // the low Qmax qubits (cost phase and their RX); the other qubits go to the
 // fewest passes that fit, each tile being the lowest qubits as carriers
@@ -1028,6 +1041,23 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
             more.push_back(p);
         }
         if (fits || blk >= rest) {
+            if (carrier_gates()) {
+                // RX on a carried low qubit may run in any strided pass that
+                // carries it (the RX gates of a layer commute and all follow
+                // the cost phase of pass 0): give each to the strided pass
+                // with the fewest gates, which takes LDS phases and VALU work
+                // off the cost pass
+                for (int q = 0; q < Qmax; ++q) {
+                    int best = -1;
+                    for (int k2 = 0; k2 < npass; ++k2)
+                        if (q < Qmax - want[k2] && (best < 0 || more[k2].gates.size() < more[best].gates.size()))
+                            best = k2;
+                    if (best < 0)
+                        continue;
+                    more[best].gates.push_back(q);
+                    ps[0].gates.erase(std::find(ps[0].gates.begin(), ps[0].gates.end(), q));
+                }
+            }
             ps.insert(ps.end(), more.begin(), more.end());
             return ps;
         }
