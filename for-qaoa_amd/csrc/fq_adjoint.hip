@@ -427,14 +427,8 @@ static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const d
     const size_t lds = pair_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q);
     const void *fn = reinterpret_cast<const void *>(&k_pair_pass<R, A_LOG, COSTBW>);
     if (lds > 65536) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               160 * 1024 - 1024);
-            if (e != hipSuccess)
-                return e;
-            attr_set = true;
-        }
+        if (hipError_t e = allow_large_lds(fn))
+            return e;
     }
     const unsigned threads = 1u << a.nthr;
     const int per_cu = occupancy_per_cu(fn, (int)threads, lds);

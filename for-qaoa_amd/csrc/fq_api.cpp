@@ -275,12 +275,11 @@ static int geom_runs(const Geom &g)
 // the strided passes); FQ_TILE_ORDER (environment) selects another.
 static uint32_t tile_order()
 {
-    static int order = -1;
-    if (order < 0) {
+    static const uint32_t order = [] {
         const char *e = getenv("FQ_TILE_ORDER");
-        order = e ? std::max(0, std::min(3, atoi(e))) : 2;
-    }
-    return (uint32_t)order;
+        return (uint32_t)(e ? std::max(0, std::min(3, atoi(e))) : 2);
+    }();
+    return order;
 }
 
 static void geom_to_args(const Geom &g, TileArgs &a)
@@ -929,11 +928,10 @@ static int carriers_for(int dtype) { return dtype == FQ_C128 ? 3 : 4; }
 // FQ_TILE_Q (environment) may lower it (smaller tiles: two work-groups per CU).
 static int pass_tile_q(int dtype)
 {
-    static int env_q = -1;
-    if (env_q < 0) {
+    static const int env_q = [] {
         const char *e = getenv("FQ_TILE_Q");
-        env_q = e ? atoi(e) : 0;
-    }
+        return e ? atoi(e) : 0;
+    }();
     const int mx = tile_max_q(dtype);
     return env_q > 0 && env_q < mx ? env_q : mx;
 }
@@ -948,11 +946,10 @@ struct FusedPass {
 // 0 for contiguous blocks.
 static int pass_block()
 {
-    static int blk = -1;
-    if (blk < 0) {
+    static const int blk = [] {
         const char *e = getenv("FQ_PASS_BLOCK");
-        blk = e ? std::max(0, atoi(e)) : 2;
-    }
+        return e ? std::max(0, atoi(e)) : 2;
+    }();
     return blk;
 }
 
@@ -961,12 +958,11 @@ static int pass_block()
 // low qubit's RX in pass 0.
 static bool carrier_gates()
 {
-    static int on = -1;
-    if (on < 0) {
+    static const bool on = [] {
         const char *e = getenv("FQ_CARRIER_GATES");
-        on = e ? (atoi(e) != 0) : 1;
-    }
-    return on != 0;
+        return e ? atoi(e) != 0 : true;
+    }();
+    return on;
 }
 
 // The tile passes of one fused layer, covering every qubit once.  Pass 0 is

@@ -107,6 +107,8 @@ struct SwapArgs {
 // LDS (at least 1), cached per (fn, threads, lds): the runtime query costs
 // tens of microseconds and the persistent launchers ask on every launch.
 int occupancy_per_cu(const void *fn, int threads, size_t lds);
+// Lets kernel fn take up to 159 KiB of dynamic LDS (set once per kernel).
+hipError_t allow_large_lds(const void *fn);
 // Compute units of the current device (cached).
 int device_cu_count();
 

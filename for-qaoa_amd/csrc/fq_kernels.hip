@@ -912,6 +912,19 @@ int device_cu_count()
 
 static int device_cus() { return device_cu_count(); }
 
+hipError_t allow_large_lds(const void *fn)
+{
+    static std::mutex mu;
+    static std::map<const void *, hipError_t> done;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(fn);
+    if (it != done.end())
+        return it->second;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    done[fn] = e;
+    return e;
+}
+
 int occupancy_per_cu(const void *fn, int threads, size_t lds)
 {
     struct Key {
@@ -951,14 +964,8 @@ static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab
     const size_t lds = GATES && a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
     const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES>);
     if (lds > 65536) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               160 * 1024 - 1024);
-            if (e != hipSuccess)
-                return e;
-            attr_set = true;
-        }
+        if (hipError_t e = allow_large_lds(fn))
+            return e;
     }
     // persistent grid: as many work-groups as can be resident at once
     const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
@@ -982,14 +989,8 @@ static hipError_t launch_mirror_m(void *sv, const TileArgs &a, hipStream_t strea
     const size_t lds = a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
     const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, false, false, true, MIRROR>);
     if (lds > 65536) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               160 * 1024 - 1024);
-            if (e != hipSuccess)
-                return e;
-            attr_set = true;
-        }
+        if (hipError_t e = allow_large_lds(fn))
+            return e;
     }
     const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
     const uint64_t resident = (uint64_t)per_cu * device_cus();
@@ -1173,14 +1174,8 @@ static hipError_t launch_swap_pipe(void *sv, const SwapArgs &a, hipStream_t stre
     const size_t lds = 2 * (size_t)side * (((size_t)side << a.m) + 1) * sizeof(T);
     const void *fn = reinterpret_cast<const void *>(&k_swap_pipe<T, E>);
     if (lds > 65536) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               160 * 1024 - 1024);
-            if (e != hipSuccess)
-                return e;
-            attr_set = true;
-        }
+        if (hipError_t e = allow_large_lds(fn))
+            return e;
     }
     const uint64_t nunits = a.npairs << a.nr;
     const int per_cu = occupancy_per_cu(fn, 256, lds);

@@ -400,14 +400,8 @@ static hipError_t launch_slab_pass_t(void *sv, const TileArgs &a, const SlotPara
     const size_t lds = a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
     const void *fn = reinterpret_cast<const void *>(&k_slab_pass<R, A_LOG, COST, MIRROR>);
     if (lds > 65536) {
-        static bool attr_set = false;
-        if (!attr_set) {
-            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               160 * 1024 - 1024);
-            if (e != hipSuccess)
-                return e;
-            attr_set = true;
-        }
+        if (hipError_t e = allow_large_lds(fn))
+            return e;
     }
     const unsigned threads = 1u << a.nthr;
     const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
