@@ -1495,10 +1495,25 @@ extern "C" int fq_state_copy_out(fq_state *st, void *host, size_t bytes, int log
     std::vector<char> tmp(bytes);
     FQ_HIP(hipMemcpyAsync(tmp.data(), st->sv, bytes, hipMemcpyDeviceToHost, st->stream));
     FQ_HIP(hipStreamSynchronize(st->stream));
+    // physical -> logical index, one table lookup per index byte
+    const int nbytes_idx = (st->N + 7) / 8;
+    std::vector<uint64_t> tab((size_t)nbytes_idx * 256, 0);
+    for (int k = 0; k < nbytes_idx; ++k)
+        for (int v = 0; v < 256; ++v) {
+            uint64_t L = 0;
+            for (int i = 0; i < 8 && 8 * k + i < st->N; ++i)
+                if ((v >> i) & 1)
+                    L |= 1ull << st->qmap[8 * k + i];
+            tab[(size_t)k * 256 + v] = L;
+        }
     const uint64_t n = 1ull << st->N;
     char *dst = (char *)host;
-    for (uint64_t b = 0; b < n; ++b)
-        memcpy(dst + to_logical_index(st, b) * st->elem, tmp.data() + b * st->elem, st->elem);
+    for (uint64_t b = 0; b < n; ++b) {
+        uint64_t L = 0;
+        for (int k = 0; k < nbytes_idx; ++k)
+            L |= tab[(size_t)k * 256 + ((b >> (8 * k)) & 255)];
+        memcpy(dst + L * st->elem, tmp.data() + b * st->elem, st->elem);
+    }
     return FQ_OK;
 }
 
@@ -1554,8 +1569,24 @@ extern "C" int fq_state_clone(fq_state *dst, const fq_state *src)
         return rc;
     if (dst->N != src->N || dst->dtype != src->dtype)
         return set_err(FQ_ERR_ARG, "clone needs equal N and dtype");
-    FQ_HIP(hipStreamSynchronize(src->stream));
-    FQ_HIP(hipMemcpyAsync(dst->sv, src->sv, src->bytes, hipMemcpyDeviceToDevice, dst->stream));
+    if (dst == src)
+        return FQ_OK;
+    // the copy runs on dst's stream after src's queued work; src's later work
+    // waits for the copy (it must not overwrite src while it is read)
+    hipEvent_t ev = nullptr;
+    FQ_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, src->stream);
+    if (e == hipSuccess)
+        e = hipStreamWaitEvent(dst->stream, ev, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst->sv, src->sv, src->bytes, hipMemcpyDeviceToDevice, dst->stream);
+    if (e == hipSuccess)
+        e = hipEventRecord(ev, dst->stream);
+    if (e == hipSuccess)
+        e = hipStreamWaitEvent(src->stream, ev, 0);
+    hipEventDestroy(ev);
+    if (e != hipSuccess)
+        return set_err(FQ_ERR_DEVICE, "clone: %s", hipGetErrorString(e));
     for (int p = 0; p < FQ_MAXN; ++p)
         dst->qmap[p] = src->qmap[p];
     return FQ_OK;

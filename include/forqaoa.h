@@ -152,7 +152,8 @@ int fq_state_copy_in(fq_state *st, const void *host, size_t bytes);
  * indices if logical != 0, else physical.  Waits. */
 int fq_state_gather(fq_state *st, const uint64_t *idx, size_t n, double *out, int logical);
 /* Copy amplitudes and qubit map of src into dst (same N and dtype), device to
- * device on dst's stream (after src's queued work). */
+ * device on dst's stream after src's queued work; work queued on src later
+ * starts after the copy.  Does not wait. */
 int fq_state_clone(fq_state *dst, const fq_state *src);
 /* Wait for all queued work of the state. */
 int fq_sync(fq_state *st);

@@ -403,3 +403,27 @@ def test_stream_probe_rates_are_plausible():
     print(f"stream probe 256 MiB: {r}", flush=True)
     for k in ("inplace_gbs", "read_gbs", "write_gbs"):
         assert 0.0 < r[k] < 24000.0
+
+
+def test_clone_copies_and_orders_the_two_streams():
+    # the copy must read src before work queued on src afterwards overwrites it
+    N = 24
+    edges = G.gnp(N, 0.5, 77, "uniform")
+    g, b = G.angles(4, 77)
+    gr = Graph(N, edges)
+    src = State(N, C=12, dtype="c128")
+    dst = State(N, C=12, dtype="c128")
+    try:
+        src.run(gr, g[:2], b[:2], "fused")
+        src.sync()
+        before = src.copy_out()
+        dst.clone_from(src)
+        src.run(gr, g[2:], b[2:], "fused")   # queued right behind the copy
+        after = src.copy_out()
+        got = dst.copy_out()
+        np.testing.assert_array_equal(got, before)
+        assert np.abs(after - before).max() > 1e-6
+    finally:
+        src.destroy()
+        dst.destroy()
+        gr.destroy()
