@@ -150,6 +150,24 @@ def test_run_fused_schedule_vs_oracle(dtype, name):
     assert ex == pytest.approx(O.expect_cut(sv, W), rel=TOL[dtype])
 
 
+@pytest.mark.parametrize("N", [22, 23, 24])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_fused_strided_geometries_vs_oracle(N, dtype):
+    # one strided pass with 3-6 carriers: wave lanes on new qubits out of
+    # index order (c128 N=23: lanes on qubits 13, 14, 19) and the carried
+    # qubits' RX moved out of the cost pass
+    edges = G.gnp(N, 0.5, 300 + N, "uniform")
+    g_, b_ = G.angles(2, 310 + N)
+    C = min(N, 13 if dtype == "c128" else 14)
+    sv, qmap, W = O.run_weighted(N, C, edges, g_, b_)
+    st = State(N, C=C, dtype=dtype)
+    g = Graph(N, edges)
+    st.run(g, g_, b_, "fused")
+    assert rel_err(st.copy_out(logical=True), O.to_logical(sv, qmap)) < TOL[dtype]
+    ex, _ = st.expect_cut(g)
+    assert ex == pytest.approx(O.expect_cut(sv, W), rel=TOL[dtype])
+
+
 @pytest.mark.parametrize("name", ["s4_gnp", "s6_reg3", "s7_gnp_int", "s8_gnp", "s10_reg4_c10"])
 @pytest.mark.parametrize("schedule", ["reference", "fused"])
 def test_run_vs_dense_fixtures(name, schedule):
