@@ -1264,6 +1264,33 @@ extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, co
     return FQ_OK;
 }
 
+extern "C" int fq_layer_passes(int N, int dtype, int schedule, int max_pass, int *npass, int *tile_q,
+                               int *tiles, uint64_t *gates)
+{
+    if (N < 1 || N > 36 || (dtype != FQ_C128 && dtype != FQ_C64) || !npass || !tile_q || !tiles || !gates)
+        return set_err(FQ_ERR_ARG, "bad N, dtype or null output");
+    std::vector<FusedPass> ps;
+    if (schedule == FQ_SCHED_FUSED)
+        ps = fused_passes(N, dtype);
+    else if (schedule == FQ_SCHED_Z2 && N >= 3)
+        ps = cover_passes(N, std::min(pass_tile_q(dtype), N - 1), dtype);
+    else
+        return set_err(FQ_ERR_ARG, "schedule must be FQ_SCHED_FUSED or FQ_SCHED_Z2 (N >= 3)");
+    if ((int)ps.size() > max_pass)
+        return set_err(FQ_ERR_ARG, "%zu passes do not fit max_pass=%d", ps.size(), max_pass);
+    *npass = (int)ps.size();
+    *tile_q = (int)ps[0].tile.size();
+    for (size_t p = 0; p < ps.size(); ++p) {
+        for (int i = 0; i < 16; ++i)
+            tiles[p * 16 + i] = i < (int)ps[p].tile.size() ? ps[p].tile[i] : -1;
+        uint64_t m = 0;
+        for (int q : ps[p].gates)
+            m |= 1ull << q;
+        gates[p] = m;
+    }
+    return FQ_OK;
+}
+
 // ---- observables -------------------------------------------------------------------------
 
 extern "C" int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *norm2)

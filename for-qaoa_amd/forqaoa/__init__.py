@@ -254,6 +254,24 @@ class State:
             pass
 
 
+def layer_passes(N, dtype="c128", schedule="fused"):
+    """The tile passes of one fused or half-state layer: a list of
+    (tile positions, RX targets) per pass; pass 0 applies the cost phase."""
+    mx = 8
+    n, q = ctypes.c_int(), ctypes.c_int()
+    tiles = np.full(16 * mx, -1, dtype=np.int32)
+    gates = np.zeros(mx, dtype=np.uint64)
+    check(lib().fq_layer_passes(int(N), DTYPES[dtype], SCHEDULES[schedule], mx, ctypes.byref(n), ctypes.byref(q),
+                                tiles.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                gates.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    out = []
+    for p in range(n.value):
+        t = [int(x) for x in tiles[16 * p:16 * p + q.value]]
+        g = [i for i in range(N) if (int(gates[p]) >> i) & 1]
+        out.append((t, g))
+    return out
+
+
 def stream_probe(nbytes, reps=3):
     """In-place, read-only and write-only streaming rates (GB/s of HBM traffic)
     of simple kernels over an nbytes buffer: what a pass that streams a state

@@ -100,6 +100,8 @@ SIGNATURES = {
     "fq_timing_get": (_i, [_vp, _i, _dp, _i64p, _dp]),
     "fq_timing_reset": (_i, [_vp]),
     "fq_stream_probe": (_i, [_sz, _i, _dp, _dp, _dp]),
+    "fq_layer_passes": (_i, [_i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                             ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 

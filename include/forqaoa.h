@@ -206,6 +206,15 @@ int fq_slab_collect(fq_slab *sb, int *slots, double *expect, double *norm2, int 
 /* Amplitudes of one slot (logical = physical order).  Waits. */
 int fq_slab_copy_out(fq_slab *sb, int slot, void *host, size_t bytes);
 
+/* The tile passes one layer of FQ_SCHED_FUSED or FQ_SCHED_Z2 runs for N
+ * qubits (no counterpart in the reference; host-only, no device needed):
+ * *npass passes of *tile_q positions each; pass p's tile positions (thread
+ * bits first, the last ones register bits) in tiles[16*p .. 16*p+15] (-1
+ * padded) and its RX targets as the bit mask gates[p].  Pass 0 applies the
+ * cost phase.  max_pass bounds the output arrays. */
+int fq_layer_passes(int N, int dtype, int schedule, int max_pass, int *npass, int *tile_q, int *tiles,
+                    uint64_t *gates);
+
 /* ---- timing (src/rx.cu:139-157 accumulates rx_gate_time_ms the same way) */
 
 /* Enable/disable per-launch HIP event timing on the state's stream. */

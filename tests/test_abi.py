@@ -144,3 +144,47 @@ def test_stream_probe_validates_before_allocating():
         assert L.fq_stream_probe(2 << 20, 1, ctypes.byref(d), ctypes.byref(d), ctypes.byref(d)) == _lib.FQ_ERR_BUDGET
     finally:
         _lib.check(L.fq_set_memory_budget(0))
+
+
+def _runs(pos):
+    n, i = 0, 0
+    while i < len(pos):
+        j = i + 1
+        while j < len(pos) and pos[j] == pos[j - 1] + 1:
+            j += 1
+        n, i = n + 1, j
+    return n
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("schedule", ["fused", "z2"])
+def test_layer_passes_cover_every_qubit_once(dtype, schedule):
+    # host-only: the pass plan every fused / half-state layer runs
+    from forqaoa import layer_passes
+    qmax = 13 if dtype == "c128" else 14
+    for N in range(3, 37):
+        ps = layer_passes(N, dtype, schedule)
+        Q = min(qmax, N - 1 if schedule == "z2" else N)
+        assert ps[0][0] == list(range(Q)), (N, ps[0])
+        seen = [q for _, g in ps for q in g]
+        assert sorted(seen) == list(range(N)), (N, ps)        # one RX per qubit per layer
+        a_log = max(1, min(3 if dtype == "c128" else 4, Q))
+        for k, (tile, gates) in enumerate(ps):
+            assert len(tile) == Q and len(set(tile)) == Q and set(gates) <= set(tile)
+            if k > 0:
+                c = next(i for i, q in enumerate(tile) if q != i) if tile != list(range(Q)) else Q
+                assert tile[:c] == list(range(c))                # carriers: the lowest qubits first
+            hidden = [p for p in range(N) if p not in tile]
+            assert max(_runs(hidden), _runs(tile[:Q - a_log])) <= 8
+        if schedule == "z2":
+            # qubit N-1 is the top register bit of the last pass, hidden elsewhere
+            assert ps[-1][0][-1] == N - 1
+            assert all(N - 1 not in t for t, _ in ps[:-1])
+
+
+def test_layer_passes_n30_c128():
+    from forqaoa import layer_passes
+    ps = layer_passes(30, "c128", "fused")
+    assert len(ps) == 3
+    assert ps[0][1] == list(range(5, 13))       # carried qubits 0..4 moved to the strided passes
+    assert sorted(ps[1][1] + ps[2][1]) == [0, 1, 2, 3, 4] + list(range(13, 30))
