@@ -310,13 +310,17 @@ static std::string geom_key(const Geom &g)
 static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
 {
     a.gate_mask = gate_mask;
+    // gated thread bits, lowest first: the last LDS phase gets the highest
+    // ones, so that it usually holds none of the six lane bits and the tile
+    // can be stored straight from its layout (direct_store)
     std::vector<int> pend;
-    for (int i = g.nthr - 1; i >= 0; --i)
+    for (int i = 0; i < g.nthr; ++i)
         if ((gate_mask >> i) & 1u)
             pend.push_back(i);
     a.ngroups = 0;
     a.grp_mask_w = 0;
     a.grp_gates_w = 0;
+    a.direct_store = 0;
     size_t pos = 0;
     while (pos < pend.size()) {
         if (a.ngroups >= FQ_MAX_GROUPS)
@@ -325,8 +329,9 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
         for (int r = 0; r < g.a_log && pos < pend.size(); ++r)
             bits.push_back(pend[pos++]);
         const int real = (int)bits.size();
-        // fill the phase with tile bits that receive no gate in it
-        for (int i = 0; (int)bits.size() < g.a_log && i < g.Q; ++i)
+        // fill the phase with tile bits that receive no gate in it, from the
+        // top (register bits first)
+        for (int i = g.Q - 1; (int)bits.size() < g.a_log && i >= 0; --i)
             if (std::find(bits.begin(), bits.end(), i) == bits.end())
                 bits.push_back(i);
         std::vector<int> sorted = bits;
@@ -346,6 +351,8 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
         a.grp_gates_w |= (uint32_t)(gg & 0xf) << (4 * a.ngroups);
         a.ngroups++;
     }
+    if (a.ngroups > 0 && g.nthr >= 6 && (a.grp_mask[a.ngroups - 1] & 0x3f) == 0)
+        a.direct_store = 1;
     return FQ_OK;
 }
 

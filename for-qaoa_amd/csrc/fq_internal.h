@@ -66,6 +66,7 @@ struct TileArgs {
     uint8_t grp_gates[FQ_MAX_GROUPS];  // register bits of that phase that receive RX
     uint64_t grp_mask_w;     // grp_mask[0..3], 16 bits each (read with a shift, not an index)
     uint32_t grp_gates_w;    // grp_gates[0..4], 4 bits each
+    uint32_t direct_store;   // last LDS phase holds no lane bit: store from its layout
     double rx_c, rx_s;       // cos(beta/2), sin(beta/2) (host-computed, src/rx.cu:121)
     double gamma, init_amp;  // cost phase angle, launch-control amplitude
     int has_rr;              // any register-register edge in the plan

@@ -233,6 +233,38 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         for (int g = 0; g < MAXG; ++g)
             etp[g] = g < a.ngroups ? lds_pad(deposit<QT>(t, qmask & ~(unsigned)a.grp_mask[g], Q)) : 0;
     }
+    // Direct store: when the last LDS phase holds none of the six lane bits,
+    // the tile is stored straight from that phase's registers (no write-back
+    // and re-read of the natural layout).  Element e of the tile lives at
+    // tile_base + phys(e); phys is additive over disjoint bits, so the
+    // thread's part (pe_last) and the part of each of the phase's register
+    // bits (pb_last) are computed once per launch.
+    // (full-width tiles only, and not in the read-write cost passes, whose
+    // registers are full with the prefetch and the phase factors)
+    const bool direct = GATES && MIRROR == 0 && QT > 0 && (FIRST || !COST) && a.direct_store && a.ngroups > 0;
+    uint64_t pe_last = 0;
+    uint64_t pb_last[A_LOG > 0 ? A_LOG : 1];   // position of each register bit of the last phase
+#pragma unroll
+    for (int r = 0; r < A_LOG; ++r)
+        pb_last[r] = 0;
+    if (direct) {
+        const unsigned qmask = (1u << Q) - 1u;
+        const unsigned gm = (unsigned)a.grp_mask[a.ngroups - 1];
+        const unsigned tmask = (1u << nthr) - 1u;
+        const unsigned et = (unsigned)deposit<QT>(t, qmask & ~gm, Q);
+        pe_last = unpack_runs((uint64_t)(et & tmask), a.truns);
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r)
+            if ((et >> (nthr + r)) & 1u)
+                pe_last |= 1ull << a.rpos[r];
+        unsigned g2 = gm;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r) {
+            const int b = __builtin_ctz(g2);
+            g2 &= g2 - 1u;
+            pb_last[r] = b < nthr ? unpack_runs(1ull << b, a.truns) : 1ull << a.rpos[b - nthr];
+        }
+    }
 
     const uint64_t q1 = (uint64_t)Q + 1;
     const double2 *thr_tab = COST ? ptab + (ntiles * q1) : nullptr;
@@ -385,18 +417,37 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                     for (int j = 0; j < A; ++j)
                         v[j] = lds[eg + ej[j]];
                     rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
+                    if (direct && g == a.ngroups - 1)
+                        break;
 #pragma unroll
                     for (int j = 0; j < A; ++j)
                         lds[eg + ej[j]] = v[j];
                     lds_barrier();
                 }
+                if (direct) {
+                    // every thread has read its last-phase elements before
+                    // the next tile's first LDS writes
+                    lds_barrier();
+                } else {
 #pragma unroll
-                for (int j = 0; j < A; ++j)
-                    v[j] = lds[nat + lds_pad(j << nthr)];
+                    for (int j = 0; j < A; ++j)
+                        v[j] = lds[nat + lds_pad(j << nthr)];
+                }
             }
         }
 
-        {
+        if (direct) {
+            T *const ps = sv + (tile_base(a, tile) + pe_last);
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                uint64_t o = 0;
+#pragma unroll
+                for (int r = 0; r < A_LOG; ++r)
+                    if ((j >> r) & 1)
+                        o += pb_last[r];
+                ps[o] = v[j];
+            }
+        } else {
             T *const p0 = sv + (tile_base(a, tile) + toff);
 #pragma unroll
             for (int j = 0; j < AH; ++j)
