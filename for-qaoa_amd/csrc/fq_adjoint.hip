@@ -66,7 +66,12 @@ __device__ __forceinline__ void rx_both(T *vp, T *vl, unsigned mask, R c, R s, d
 // register-register entry, and its phase is the product of the same entries
 // of ptab.  partials[2*block] and [2*block+1] receive the work-group's two
 // sums.
-template <typename R, int A_LOG, bool COSTBW>
+// Half states (run_tiles < 2^nh): the tiles with qubit N-1 = 0 only; MIRROR:
+// qubit N-1 is the top register bit and the upper register half addresses
+// tile ~tau at complemented offsets (psi(b) = psi(~b)), as in k_tile.  Every
+// element then stands for itself and its complement: the caller doubles the
+// sums.
+template <typename R, int A_LOG, bool COSTBW, bool MIRROR>
 __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict__ psi,
                                                     typename CT<R>::T *__restrict__ lam,
                                                     const TileArgs a,
@@ -82,18 +87,39 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     const int Q = a.Q, nthr = a.nthr;
     const int t = threadIdx.x;
     const int nthreads = blockDim.x;
-    const uint64_t ntiles = 1ull << a.nh;
+    static_assert(!(COSTBW && MIRROR), "the cost pass is never the mirror pass");
+    constexpr int AH = MIRROR ? A / 2 : A;
+    const uint64_t ntiles = run_tiles(a);
+    const uint64_t hmask = (1ull << a.nh) - 1ull;
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     uint64_t roff[A];
 #pragma unroll
     for (int j = 0; j < A; ++j) {
         uint64_t o = 0;
 #pragma unroll
-        for (int r = 0; r < A_LOG; ++r)
+        for (int r = 0; r < (MIRROR ? A_LOG - 1 : A_LOG); ++r)
             if ((j >> r) & 1)
                 o |= 1ull << a.rpos[r];
         roff[j] = o;
     }
+    // both states' elements of a tile in registers (upper register half of a
+    // mirror pass: the complementary tile at complemented offsets)
+    auto load_tile = [&](uint64_t tl, T *p, T *l) {
+        const uint64_t b0 = unpack_runs(tl, a.hruns) + toff;
+#pragma unroll
+        for (int j = 0; j < AH; ++j) {
+            p[j] = psi[b0 + roff[j]];
+            l[j] = lam[b0 + roff[j]];
+        }
+        if (MIRROR) {
+            const uint64_t bm = unpack_runs(~tl & hmask, a.hruns) + a.mirror_pm - toff;
+#pragma unroll
+            for (int j = AH; j < A; ++j) {
+                p[j] = psi[bm - roff[j - AH]];
+                l[j] = lam[bm - roff[j - AH]];
+            }
+        }
+    };
     const size_t lds_elems = ((size_t)1 << Q) + ((size_t)1 << Q) / 16;
     T *ldsP = reinterpret_cast<T *>(fq_smem_a);
     T *ldsL = ldsP + lds_elems;
@@ -140,12 +166,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     const uint64_t end_tile = first_tile + per < ntiles ? first_tile + per : ntiles;
     T vp[A], vl[A];
     if (first_tile < end_tile) {
-        const uint64_t b0 = unpack_runs(first_tile, a.hruns) + toff;
-#pragma unroll
-        for (int j = 0; j < A; ++j) {
-            vp[j] = psi[b0 + roff[j]];
-            vl[j] = lam[b0 + roff[j]];
-        }
+        load_tile(first_tile, vp, vl);
         if (COSTBW && row_lane) {
             crow = ctab[first_tile * q1 + lane];
             frow = ptab[first_tile * q1 + lane];
@@ -155,23 +176,14 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
         const uint64_t base = unpack_runs(tile, a.hruns) + toff;
         T wp[A], wl[A];
         if (!PREFETCH && tile != first_tile) {
-#pragma unroll
-            for (int j = 0; j < A; ++j) {
-                vp[j] = psi[base + roff[j]];
-                vl[j] = lam[base + roff[j]];
-            }
+            load_tile(tile, vp, vl);
             if (COSTBW && row_lane) {
                 crow = ctab[tile * q1 + lane];
                 frow = ptab[tile * q1 + lane];
             }
         }
         if (PREFETCH && tile + 1 < end_tile) {
-            const uint64_t b1 = unpack_runs(tile + 1, a.hruns) + toff;
-#pragma unroll
-            for (int j = 0; j < A; ++j) {
-                wp[j] = psi[b1 + roff[j]];
-                wl[j] = lam[b1 + roff[j]];
-            }
+            load_tile(tile + 1, wp, wl);
             if (COSTBW && row_lane) {
                 cnext = ctab[(tile + 1) * q1 + lane];
                 fnext = ptab[(tile + 1) * q1 + lane];
@@ -293,9 +305,17 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
             }
         }
 #pragma unroll
-        for (int j = 0; j < A; ++j) {
+        for (int j = 0; j < AH; ++j) {
             psi[base + roff[j]] = vp[j];
             lam[base + roff[j]] = vl[j];
+        }
+        if (MIRROR) {
+            const uint64_t bm = unpack_runs(~tile & hmask, a.hruns) + a.mirror_pm - toff;
+#pragma unroll
+            for (int j = AH; j < A; ++j) {
+                psi[bm - roff[j - AH]] = vp[j];
+                lam[bm - roff[j - AH]] = vl[j];
+            }
         }
         // the next tile's LDS phases start after every thread has read its
         // amplitudes of this one back
@@ -418,36 +438,39 @@ size_t pair_lds_bytes(int dtype, int Q)
 
 // Persistent grid: as many work-groups as can be resident (one partial sum
 // pair each, at most the tile count).
-template <typename R, int A_LOG, bool COSTBW>
+template <typename R, int A_LOG, bool COSTBW, bool MIRROR>
 static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const double2 *ptab,
                                 const double *ctab, double *partials, unsigned *grid_out,
                                 hipStream_t stream)
 {
     typedef typename CT<R>::T T;
     const size_t lds = pair_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q);
-    const void *fn = reinterpret_cast<const void *>(&k_pair_pass<R, A_LOG, COSTBW>);
+    const void *fn = reinterpret_cast<const void *>(&k_pair_pass<R, A_LOG, COSTBW, MIRROR>);
     if (lds > 65536) {
         if (hipError_t e = allow_large_lds(fn))
             return e;
     }
     const unsigned threads = 1u << a.nthr;
     const int per_cu = occupancy_per_cu(fn, (int)threads, lds);
-    const uint64_t tiles = 1ull << a.nh;
+    const uint64_t tiles = run_tiles(a);
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)device_cu_count();
     const unsigned grid = (unsigned)(tiles < resident ? tiles : resident);
     *grid_out = grid;
-    hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW>), dim3(grid), dim3(threads), lds, stream,
+    hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW, MIRROR>), dim3(grid), dim3(threads), lds, stream,
                        reinterpret_cast<T *>(psi), reinterpret_cast<T *>(lam), a, ptab, ctab, partials);
     return hipGetLastError();
 }
 
-hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
+hipError_t launch_pair(int dtype, int a_log, bool costbw, bool mirror, void *psi, void *lam, const TileArgs &a,
                        const double2 *ptab, const double *ctab, double *partials, unsigned *grid,
                        hipStream_t stream)
 {
-#define FQ_PAIR(R, AL)                                                                              \
-    return costbw ? launch_pair_t<R, AL, true>(psi, lam, a, ptab, ctab, partials, grid, stream)    \
-                  : launch_pair_t<R, AL, false>(psi, lam, a, ptab, ctab, partials, grid, stream);
+    if (costbw && mirror)
+        return hipErrorInvalidValue;
+#define FQ_PAIR(R, AL)                                                                                   \
+    return costbw ? launch_pair_t<R, AL, true, false>(psi, lam, a, ptab, ctab, partials, grid, stream)  \
+           : mirror ? launch_pair_t<R, AL, false, true>(psi, lam, a, ptab, ctab, partials, grid, stream) \
+                    : launch_pair_t<R, AL, false, false>(psi, lam, a, ptab, ctab, partials, grid, stream);
     if (dtype == 0) {
         switch (a_log) {
             case 1: FQ_PAIR(double, 1)
@@ -468,7 +491,7 @@ hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, 
 hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, const TileArgs &a,
                           double *partials, hipStream_t stream)
 {
-    const uint64_t tiles = 1ull << a.nh;
+    const uint64_t tiles = run_tiles(a);
     const dim3 g((unsigned)tiles), b(1u << a.nthr);
 #define FQ_CM(R, AL)                                                                                       \
     hipLaunchKernelGGL((k_cut_mul<R, AL>), g, b, 0, stream,                                               \

@@ -1372,9 +1372,17 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
         if (st->qmap[p] != p)
             return set_err(FQ_ERR_ARG, "fq_backward needs the identity qubit map (fused schedule)");
     const int N = st->N;
+    // Half states: psi_P and lambda = Cut psi_P satisfy x(b) = x(~b) (Cut(b) =
+    // Cut(~b)), and so do both states walked back through U^+ M^+, so for
+    // N >= 3 every pass runs on the half z_{N-1} = 0 (the last one as the
+    // mirror pass) and every sum counts each element for itself and its
+    // complement: the sums are doubled.  The state's upper half is left
+    // stale (the state is consumed).
+    const bool z2 = N >= 3;
+    const double mult = z2 ? 2.0 : 1.0;
     // 2^12-amplitude tiles of psi and lambda share the LDS (2 x 68 KiB);
     // 512 threads x 8 amplitudes of each state
-    const int Qb = std::min(12, N);
+    const int Qb = std::min(12, z2 ? N - 1 : N);
     const std::vector<FusedPass> ps = cover_passes(N, Qb, st->dtype);
     const size_t npass = ps.size();
     if (int rc = ensure_aux(st, 1 + npass * (size_t)P))
@@ -1382,15 +1390,17 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
 
     // lambda = Cut * psi_P, <C> into slot 0
     {
-        const int Q = std::min(12, N);
+        const int Q = Qb;
         const int a_log = std::max(1, std::min(4, Q));
         Geom geom = make_geom(N, low_tile(Q), a_log);
         TileArgs a;
         geom_to_args(geom, a);
+        if (z2)
+            a.ntiles_run = 1ull << (geom.nh - 1);   // qubit N-1 is the top hidden bit
         if (int rc = get_plan(g, geom, &a.plan, st->stream))
             return rc;
         FQ_HIP(launch_cut_mul(st->dtype, a_log, st->sv, st->aux, a, st->d_part, st->stream));
-        FQ_HIP(launch_finalize(st->d_part, 1ull << a.nh, st->d_grad, st->stream));
+        FQ_HIP(launch_finalize(st->d_part, run_tiles(a), st->d_grad, st->stream));
     }
     for (int k = P - 1; k >= 0; --k) {
         for (int i = (int)npass - 1; i >= 0; --i) {
@@ -1409,6 +1419,17 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
                         gm |= 1u << ti;
             if (int rc = set_gates(a, geom, gm))
                 return rc;
+            const bool mirror = z2 && i + 1 == (int)npass;
+            if (z2) {
+                const bool ok = mirror ? ps[i].tile.back() == N - 1 && !costbw
+                                       : geom.nh >= 1 && geom.hpos[geom.nh - 1] == N - 1;
+                if (!ok)
+                    return set_err(FQ_ERR_ARG, "internal: adjoint tile does not fit the half-state pass");
+                a.ntiles_run = 1ull << (geom.nh - 1);
+                if (mirror)
+                    for (int ti = 0; ti + 1 < Q; ++ti)
+                        a.mirror_pm += 1ull << ps[i].tile[ti];
+            }
             double sn, cs;
             sincos(-betas[k] / 2, &sn, &cs);
             a.rx_c = cs;
@@ -1422,9 +1443,9 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
                     return rc;
             }
             double *slot = st->d_grad + 2 * (1 + (size_t)k * npass + i);
-            int rc = timed(st, FQ_K_ADJOINT, 4.0 * (double)st->bytes, [&] {
+            int rc = timed(st, FQ_K_ADJOINT, 4.0 * (double)st->bytes / mult, [&] {
                 unsigned grid = 0;
-                hipError_t e = launch_pair(st->dtype, a_log, costbw, st->sv, st->aux, a,
+                hipError_t e = launch_pair(st->dtype, a_log, costbw, mirror, st->sv, st->aux, a,
                                            costbw ? st->d_phase : nullptr, ctab, st->d_part, &grid,
                                            st->stream);
                 if (e != hipSuccess)
@@ -1440,13 +1461,13 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
                           st->stream));
     FQ_HIP(hipStreamSynchronize(st->stream));
     if (expect)
-        *expect = h[0];
+        *expect = mult * h[0];
     for (int k = 0; k < P; ++k) {
         double gb = 0.0;
         for (size_t i = 0; i < npass; ++i)
             gb += h[2 * (1 + k * npass + i)];
-        grad_betas[k] = gb;
-        grad_gammas[k] = -h[2 * (1 + k * npass) + 1];
+        grad_betas[k] = mult * gb;
+        grad_gammas[k] = -mult * h[2 * (1 + k * npass) + 1];
     }
     return FQ_OK;
 }

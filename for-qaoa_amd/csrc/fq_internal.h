@@ -144,7 +144,8 @@ hipError_t launch_finalize(const double *partials, uint64_t n, double *out2, hip
 // grid receives the number of work-groups (= partial sum pairs written).
 // With costbw, ptab and ctab are the phase and cut tables of one
 // launch_phase_table call.
-hipError_t launch_pair(int dtype, int a_log, bool costbw, void *psi, void *lam, const TileArgs &a,
+// mirror: the half-state mirror pass (qubit N-1 the top register bit).
+hipError_t launch_pair(int dtype, int a_log, bool costbw, bool mirror, void *psi, void *lam, const TileArgs &a,
                        const double2 *ptab, const double *ctab, double *partials, unsigned *grid,
                        hipStream_t stream);
 hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, const TileArgs &a,

@@ -164,7 +164,8 @@ int fq_sync(fq_state *st);
 /* d<C>/dgamma_k and d<C>/dbeta_k of the P-layer evaluation by reverse
  * uncompute (no checkpoints): st must hold the state fq_run(st, g, P, gammas,
  * betas, FQ_SCHED_FUSED or FQ_SCHED_Z2) produced (identity qubit map).  The state is consumed
- * (walked back to the launch-control state); a second state-sized buffer is
+ * (its amplitudes are unspecified afterwards: for N >= 3 the walk back runs on
+ * the half z_{N-1} = 0, psi(b) = psi(~b)); a second state-sized buffer is
  * allocated on first use.  expect (may be NULL) receives <C>.  FP64
  * accumulation, fixed reduction order.  Waits. */
 int fq_backward(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
