@@ -195,7 +195,7 @@ def serving_bench(n_sat=10000, n_base=150, n_lat=600, seed=20251207):
 def adjoint_bench(n_sets=16, seed=2026):
     """The adjoint workload: N=26, P=8, seeded 4-regular graph, complex64;
     for each of 16 random angle sets one forward evaluation (half-state
-    schedule) and one backward pass (fq_backward).  Reports gradients/s and
+    schedule) and one backward pass (fq_backward, on half states).  Reports gradients/s and
     the reverse pass's achieved bandwidth (4S per pair pass)."""
     import numpy as np
     from forqaoa import FQ_K_ADJOINT, FQ_K_FUSED, Graph, State
