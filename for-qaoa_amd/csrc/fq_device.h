@@ -95,6 +95,45 @@ __device__ __forceinline__ int deposit_runs(int x, unsigned mask)
     return e;
 }
 
+// Exponent over gamma of entry gid of a cost launch's phase table (the
+// layout of k_phase_table): per tile the coefficients c_i(zH) (i < Q) and
+// K(zH) - W/2, then the per-thread and per-register entries and the 16
+// register-register entries.  Returns false past the end of the table.
+__device__ __forceinline__ bool phase_coef(const TileArgs &a, const PlanDev &p, int a_log, uint64_t gid,
+                                           double &coef)
+{
+    const uint64_t q1 = (uint64_t)a.Q + 1;
+    const uint64_t ntile_e = run_tiles(a) * q1;
+    const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
+    if (gid < ntile_e) {
+        const uint64_t tile = gid / q1;
+        const int i = (int)(gid - tile * q1);
+        if (i < a.Q) {
+            double acc = 0.0;
+            for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
+                acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
+            coef = p.cH[i] - 2.0 * acc;
+        } else {
+            double K = 0.0;
+            for (int e = 0; e < p.n_hh; ++e)
+                K += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
+            for (int h = 0; h < a.nh; ++h)
+                K += p.omega[h] * (double)((tile >> h) & 1ull);
+            coef = K - 0.5 * p.wtot;
+        }
+        return true;
+    }
+    if (gid < ntile_e + nthr_e) {
+        coef = p.thr[gid - ntile_e];
+        return true;
+    }
+    if (gid < ntile_e + nthr_e + 16) {
+        coef = p.rr[gid - ntile_e - nthr_e];
+        return true;
+    }
+    return false;
+}
+
 // Barrier that orders LDS accesses only: outstanding global loads (the next
 // tile's prefetch) stay in flight across it.
 __device__ __forceinline__ void lds_barrier()

@@ -88,35 +88,10 @@ __global__ void __launch_bounds__(256) k_phase_table(const TileArgs a, int a_log
                                                      double2 *__restrict__ tab,
                                                      double *__restrict__ cut)
 {
-    const PlanDev &p = a.plan;
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t q1 = (uint64_t)a.Q + 1;
-    const uint64_t ntile_e = run_tiles(a) * q1;
-    const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
     double coef;
-    if (gid < ntile_e) {
-        const uint64_t tile = gid / q1;
-        const int i = (int)(gid - tile * q1);
-        if (i < a.Q) {
-            double acc = 0.0;
-            for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
-                acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
-            coef = p.cH[i] - 2.0 * acc;
-        } else {
-            double K = 0.0;
-            for (int e = 0; e < p.n_hh; ++e)
-                K += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
-            for (int h = 0; h < a.nh; ++h)
-                K += p.omega[h] * (double)((tile >> h) & 1ull);
-            coef = K - 0.5 * p.wtot;
-        }
-    } else if (gid < ntile_e + nthr_e) {
-        coef = p.thr[gid - ntile_e];
-    } else if (gid < ntile_e + nthr_e + 16) {
-        coef = p.rr[gid - ntile_e - nthr_e];
-    } else {
+    if (!phase_coef(a, a.plan, a_log, gid, coef))
         return;
-    }
     double sn, cs;
     sincos(a.gamma * coef, &sn, &cs);
     tab[gid] = make_double2(cs, sn);

@@ -24,35 +24,11 @@ __global__ void __launch_bounds__(256) k_slab_phase_table(const TileArgs a, int 
     const SlotParams &sp = slots[blockIdx.y];
     if (!sp.active)
         return;
-    const PlanDev &p = sp.plan;
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t q1 = (uint64_t)a.Q + 1;
-    const uint64_t ntile_e = run_tiles(a) * q1;
-    const uint64_t nthr_e = (uint64_t)(1 + a_log) << a.nthr;
-    double angle;
-    if (gid < ntile_e) {
-        const uint64_t tile = gid / q1;
-        const int i = (int)(gid - tile * q1);
-        if (i < a.Q) {
-            double acc = 0.0;
-            for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
-                acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
-            angle = sp.gamma * (p.cH[i] - 2.0 * acc);
-        } else {
-            double K = 0.0;
-            for (int e = 0; e < p.n_hh; ++e)
-                K += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
-            for (int h = 0; h < a.nh; ++h)
-                K += p.omega[h] * (double)((tile >> h) & 1ull);
-            angle = sp.gamma * (K - 0.5 * p.wtot);
-        }
-    } else if (gid < ntile_e + nthr_e) {
-        angle = sp.gamma * p.thr[gid - ntile_e];
-    } else if (gid < ntile_e + nthr_e + 16) {
-        angle = sp.gamma * p.rr[gid - ntile_e - nthr_e];
-    } else {
+    double coef;
+    if (!phase_coef(a, sp.plan, a_log, gid, coef))
         return;
-    }
+    const double angle = sp.gamma * coef;
     double sn, cs;
     sincos(angle, &sn, &cs);
     tabs[stride * blockIdx.y + gid] = make_double2(cs, sn);
