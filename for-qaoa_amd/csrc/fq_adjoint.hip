@@ -369,20 +369,7 @@ __global__ void __launch_bounds__(256) k_cut_mul(const typename CT<R>::T *__rest
     const int nthreads = blockDim.x;
     const uint64_t tile = blockIdx.x;
     const uint64_t base = unpack_runs(tile, a.hruns) + unpack_runs((uint64_t)t, a.truns);
-    for (int i = t; i < a.Q; i += nthreads) {
-        double acc = 0.0;
-        for (int e = a.plan.csr_off[i]; e < a.plan.csr_off[i + 1]; ++e)
-            acc += a.plan.csr_w[e] * (double)((tile >> a.plan.csr_h[e]) & 1ull);
-        s_c[i] = a.plan.cH[i] - 2.0 * acc;
-    }
-    if (t == 0) {
-        double K = 0.0;
-        for (int e = 0; e < a.plan.n_hh; ++e)
-            K += a.plan.hh_w[e] * (double)(((tile >> a.plan.hh_u[e]) ^ (tile >> a.plan.hh_v[e])) & 1ull);
-        for (int h = 0; h < a.nh; ++h)
-            K += a.plan.omega[h] * (double)((tile >> h) & 1ull);
-        s_K = K;
-    }
+    cut_tile_prologue(a.plan, a.Q, a.nh, tile, s_c, &s_K, t, nthreads);
     __syncthreads();
     double cbase = s_K + a.plan.thr[t];
     for (int i = 0; i < a.nthr; ++i)

@@ -134,6 +134,44 @@ __device__ __forceinline__ bool phase_coef(const TileArgs &a, const PlanDev &p, 
     return false;
 }
 
+// Per-tile part of the cut decomposition, computed by at most two waves:
+// c_i(zH) for every tile bit (lanes i < Q) and K(zH) over the tile index's nh
+// hidden bits (one wave, butterfly reduction, lane 0 writes).  Published by
+// the caller's barrier.
+__device__ __forceinline__ void cut_tile_prologue(const PlanDev &p, int Q, int nh, uint64_t tile, double *s_c,
+                                                  double *s_K, int t, int nthreads)
+{
+    for (int i = t; i < Q; i += nthreads) {
+        double acc = 0.0;
+        for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
+            acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
+        s_c[i] = p.cH[i] - 2.0 * acc;
+    }
+    if (nthreads >= 64) {
+        const int kw = nthreads >= 128 ? 64 : 0;
+        if (t >= kw && t < kw + 64) {
+            const int lane = t - kw;
+            double acc = 0.0;
+            for (int e = lane; e < p.n_hh; e += 64)
+                acc += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
+            for (int h = lane; h < nh; h += 64)
+                acc += p.omega[h] * (double)((tile >> h) & 1ull);
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1)
+                acc += __shfl_xor(acc, off, 64);
+            if (lane == 0)
+                *s_K = acc;
+        }
+    } else if (t == 0) {
+        double acc = 0.0;
+        for (int e = 0; e < p.n_hh; ++e)
+            acc += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
+        for (int h = 0; h < nh; ++h)
+            acc += p.omega[h] * (double)((tile >> h) & 1ull);
+        *s_K = acc;
+    }
+}
+
 // Barrier that orders LDS accesses only: outstanding global loads (the next
 // tile's prefetch) stay in flight across it.
 __device__ __forceinline__ void lds_barrier()

@@ -34,44 +34,6 @@
 
 namespace fq {
 
-// Per-tile part of the cut decomposition, computed by at most two waves:
-// c_i(zH) for every tile bit (lanes i < Q) and K(zH) (one wave, butterfly
-// reduction, lane 0 writes).  Published by the caller's barrier.
-__device__ __forceinline__ void cut_tile_prologue(const TileArgs &a, uint64_t tile, double *s_c,
-                                                  double *s_K, int t, int nthreads)
-{
-    const PlanDev &p = a.plan;
-    for (int i = t; i < a.Q; i += nthreads) {
-        double acc = 0.0;
-        for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
-            acc += p.csr_w[e] * (double)((tile >> p.csr_h[e]) & 1ull);
-        s_c[i] = p.cH[i] - 2.0 * acc;
-    }
-    if (nthreads >= 64) {
-        const int kw = nthreads >= 128 ? 64 : 0;
-        if (t >= kw && t < kw + 64) {
-            const int lane = t - kw;
-            double acc = 0.0;
-            for (int e = lane; e < p.n_hh; e += 64)
-                acc += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
-            for (int h = lane; h < a.nh; h += 64)
-                acc += p.omega[h] * (double)((tile >> h) & 1ull);
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1)
-                acc += __shfl_xor(acc, off, 64);
-            if (lane == 0)
-                *s_K = acc;
-        }
-    } else if (t == 0) {
-        double acc = 0.0;
-        for (int e = 0; e < p.n_hh; ++e)
-            acc += p.hh_w[e] * (double)(((tile >> p.hh_u[e]) ^ (tile >> p.hh_v[e])) & 1ull);
-        for (int h = 0; h < a.nh; ++h)
-            acc += p.omega[h] * (double)((tile >> h) & 1ull);
-        *s_K = acc;
-    }
-}
-
 // Phase table of one cost-phase launch (the tile kernel does no transcendental
 // work).  Layout: for each tile, Q+1 factors
 //   tab[tile*(Q+1) + i] = exp(i*gamma*c_i(zH))          i < Q
@@ -482,7 +444,7 @@ __global__ void __launch_bounds__(256) k_expect(const typename CT<R>::T *__restr
                 o |= 1ull << a.rpos[r];
         v[j] = p0[o];
     }
-    cut_tile_prologue(a, tile, s_c, &s_K, t, nthreads);
+    cut_tile_prologue(a.plan, a.Q, a.nh, tile, s_c, &s_K, t, nthreads);
     __syncthreads();
     double cbase = s_K + a.plan.thr[t];
     for (int i = 0; i < a.nthr; ++i)

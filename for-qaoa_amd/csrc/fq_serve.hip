@@ -265,20 +265,7 @@ __global__ void __launch_bounds__(256) k_slab_expect(const typename CT<R>::T *__
     const PlanDev &p = sp.eplan;
     const uint64_t lt = tile & ((1ull << slot_shift) - 1ull);
     const uint64_t base = unpack_runs(tile, a.hruns) + unpack_runs((uint64_t)t, a.truns);
-    for (int i = t; i < a.Q; i += nthreads) {
-        double acc = 0.0;
-        for (int e = p.csr_off[i]; e < p.csr_off[i + 1]; ++e)
-            acc += p.csr_w[e] * (double)((lt >> p.csr_h[e]) & 1ull);
-        s_c[i] = p.cH[i] - 2.0 * acc;
-    }
-    if (t == 0) {
-        double K = 0.0;
-        for (int e = 0; e < p.n_hh; ++e)
-            K += p.hh_w[e] * (double)(((lt >> p.hh_u[e]) ^ (lt >> p.hh_v[e])) & 1ull);
-        for (int h = 0; h < slot_shift; ++h)
-            K += p.omega[h] * (double)((lt >> h) & 1ull);
-        s_K = K;
-    }
+    cut_tile_prologue(p, a.Q, slot_shift, lt, s_c, &s_K, t, nthreads);
     __syncthreads();
     double cbase = s_K + p.thr[t];
     for (int i = 0; i < a.nthr; ++i)
