@@ -862,10 +862,10 @@ enum { Z2_OFF = 0, Z2_LOWER = 1, Z2_MIRROR = 2, Z2_EXPAND = 3 };
 // Launch arguments of one tile pass of an N-qubit state: tile positions
 // (thread bits first), RX gate positions and the half-state mode.
 static int pass_args(int N, int dtype, const std::vector<int> &tile, const std::vector<int> &gates, bool cost,
-                     int z2, TileArgs &a, Geom &geom)
+                     int z2, TileArgs &a, Geom &geom, int a_log_override = 0)
 {
     const int Q = (int)tile.size();
-    const int a_log = tile_a_log(dtype, Q);
+    const int a_log = a_log_override > 0 ? a_log_override : tile_a_log(dtype, Q);
     geom = make_geom(N, tile, a_log);
     if (geom_runs(geom) > FQ_MAX_RUNS)
         return set_err(FQ_ERR_ARG, "internal: tile geometry needs more than %d bit runs", FQ_MAX_RUNS);
@@ -1406,30 +1406,14 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
         for (int i = (int)npass - 1; i >= 0; --i) {
             const bool costbw = i == 0;
             const int Q = (int)ps[i].tile.size();
+            // 512 threads x 8 amplitudes of each state
             const int a_log = std::max(1, std::min(3, Q));
-            Geom geom = make_geom(N, ps[i].tile, a_log);
-            if (geom_runs(geom) > FQ_MAX_RUNS)
-                return set_err(FQ_ERR_ARG, "internal: adjoint tile needs too many bit runs");
-            TileArgs a;
-            geom_to_args(geom, a);
-            uint32_t gm = 0;
-            for (int q : ps[i].gates)
-                for (int ti = 0; ti < Q; ++ti)
-                    if (ps[i].tile[ti] == q)
-                        gm |= 1u << ti;
-            if (int rc = set_gates(a, geom, gm))
-                return rc;
             const bool mirror = z2 && i + 1 == (int)npass;
-            if (z2) {
-                const bool ok = mirror ? ps[i].tile.back() == N - 1 && !costbw
-                                       : geom.nh >= 1 && geom.hpos[geom.nh - 1] == N - 1;
-                if (!ok)
-                    return set_err(FQ_ERR_ARG, "internal: adjoint tile does not fit the half-state pass");
-                a.ntiles_run = 1ull << (geom.nh - 1);
-                if (mirror)
-                    for (int ti = 0; ti + 1 < Q; ++ti)
-                        a.mirror_pm += 1ull << ps[i].tile[ti];
-            }
+            Geom geom;
+            TileArgs a;
+            if (int rc = pass_args(N, st->dtype, ps[i].tile, ps[i].gates, costbw,
+                                   z2 ? (mirror ? Z2_MIRROR : Z2_LOWER) : Z2_OFF, a, geom, a_log))
+                return rc;
             double sn, cs;
             sincos(-betas[k] / 2, &sn, &cs);
             a.rx_c = cs;
