@@ -1,7 +1,7 @@
 """GPU parity: the HIP library (through its C ABI) against the CPU oracle on
 the same seeded inputs, and against closed forms at full size.
 
-Tolerances (BASELINE north star): complex128 results within 1e-12 relative
+Tolerances: complex128 results within 1e-12 relative
 of the oracle (states: ||psi_gpu - psi_oracle||_2 / ||psi_oracle||_2; <C>:
 relative); complex64 results within 1e-5 of the complex128 oracle.  Integer
 work (qubit maps, index permutations) is exact.
