@@ -134,3 +134,46 @@ def test_torch_autograd_function():
         b2 = bt + 0.01 * bt.grad
     E2 = qaoa_expectation(g2, b2, gr, dtype="c128")
     assert float(E2) > float(E)
+
+
+def _random_adjoint_cases(n=16, seed=20251203):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        N = int(rng.integers(3, 19))
+        P = int(rng.integers(1, 5))
+        dtype = ["c128", "c64"][i % 2]
+        p = float(rng.uniform(0.3, 0.9))
+        kind = ["uniform", "int16", "unit"][int(rng.integers(0, 3))]
+        out.append((i, N, P, dtype, p, kind))
+    return out
+
+
+@pytest.mark.parametrize("case", _random_adjoint_cases(), ids=lambda c: f"{c[0]}-N{c[1]}-P{c[2]}-{c[3]}")
+def test_adjoint_random_configurations(case):
+    # seeded graphs and depths across the tile-width boundaries: the half-state
+    # walk back (mirror pass, doubled sums) against central differences of the
+    # oracle's <C>
+    i, N, P, dtype, p, kind = case
+    print(f"case {i}: N={N} P={P} {dtype} G(N,{p:.2f}) {kind}", flush=True)
+    edges = G.gnp(N, p, 500 + i, kind)
+    g, b = G.angles(P, 600 + i)
+    st = State(N, C=min(N, 10), dtype=dtype)
+    gr = Graph(N, edges)
+    try:
+        st.run(gr, g, b, "z2" if N >= 3 else "fused")
+        e, gg, gb = st.backward(gr, g, b)
+        scale_e = max(1.0, abs(oracle_expect(N, edges, g, b)))
+        assert abs(e - oracle_expect(N, edges, g, b)) / scale_e < (1e-12 if dtype == "c128" else 1e-5)
+        # the phase angles scale with the weights (integer weights up to 16):
+        # the difference step shrinks with them, so the central difference's
+        # O(h^2) truncation stays below the tolerance
+        wmax = max([1.0] + [abs(w) for _, _, w in edges])
+        fg, fb = central_fd(lambda x, y: oracle_expect(N, edges, x, y), g, b, 1e-4 / wmax)
+        scale = max(1.0, float(np.max(np.abs(np.concatenate([fg, fb])))))
+        err = float(np.max(np.abs(np.concatenate([gg - fg, gb - fb])))) / scale
+        print(f"  max |g_adj - g_fd| / max(1,|g|) = {err:.2e}", flush=True)
+        assert err < (1e-6 if dtype == "c128" else 1e-4)
+    finally:
+        st.destroy()
+        gr.destroy()
