@@ -125,3 +125,50 @@ def test_one_at_a_time_baseline_agrees():
     one_at_a_time(stream, stream.requests, dtype="c128", schedule="reference")
     for r in stream.requests:
         assert r.expect == pytest.approx(done[r.rid], rel=1e-12)
+
+
+def _random_slab_cases(n=12, seed=20251204):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        N = int(rng.integers(3, 25))
+        K = int([2, 4, 8][int(rng.integers(0, 3))])
+        schedule = ["fused", "z2"][int(rng.integers(0, 2))]
+        dtype = ["c128", "c64"][i % 2]
+        out.append((i, N, K, schedule, dtype))
+    return out
+
+
+@pytest.mark.parametrize("case", _random_slab_cases(), ids=lambda c: f"{c[0]}-N{c[1]}-K{c[2]}-{c[3]}-{c[4]}")
+def test_slab_random_sizes_match_solo(case):
+    # seeded qubit counts, slot counts and depths: every request's <C> from a
+    # packed slab equals the same request run alone
+    i, N, K, schedule, dtype = case
+    print(f"case {i}: N={N} K={K} {schedule} {dtype}", flush=True)
+    rng = np.random.default_rng(700 + i)
+    sb = Slab(N, K, dtype, schedule)
+    graphs = []
+    try:
+        reqs = []
+        for k in range(min(K, 3)):
+            edges = G.gnp(N, float(rng.uniform(0.3, 0.9)), 800 + 10 * i + k, "uniform")
+            P = int(rng.integers(1, 5))
+            g, b = G.angles(P, 900 + 10 * i + k)
+            gr = Graph(N, edges)
+            graphs.append(gr)
+            r = type("R", (), {"gammas": g, "betas": b, "edges": edges, "expect": None})()
+            reqs.append(r)
+            sb.assign(k, gr, r)
+        for _ in range(8):
+            if all(r.expect is not None for r in reqs):
+                break
+            sb.step()
+            sb.poll()
+            sb.collect()
+        for r in reqs:
+            e_solo, _ = solo(N, r.edges, r.gammas, r.betas, dtype)
+            assert r.expect == pytest.approx(e_solo, rel=TOL[dtype], abs=TOL[dtype])
+    finally:
+        sb.destroy()
+        for gr in graphs:
+            gr.destroy()
