@@ -1356,8 +1356,9 @@ static int ensure_aux(fq_state *st, size_t grad_slots)
 }
 
 // Reverse-uncompute gradient of <C> (fq_adjoint.hip).  The state must hold
-// psi_P of fq_run(FUSED) with the same graph and angles (identity qubit map);
-// it is consumed (walked back to |+>^N).
+// psi_P of fq_run(FUSED or Z2) with the same graph and angles (identity qubit
+// map); it is consumed (for N >= 3 its lower half is walked back to |+>^N and
+// its upper half is left as it was).
 extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gammas,
                            const double *betas, double *expect, double *grad_gammas,
                            double *grad_betas)
