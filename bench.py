@@ -78,7 +78,7 @@ def measure(name, dtype, schedule, steps, warmup, C=None, dominant=None):
     # the reference schedule relabels g in place, so g matches the final layout
     ex, n2 = st.expect_cut(g)
     idx, prob = st.topk(8)
-    res = dict(workload=name, N=N, P=P, C=C, dtype=dtype, schedule=schedule, steps=steps,
+    res = dict(workload=name, desc=w["desc"], N=N, P=P, C=C, dtype=dtype, schedule=schedule, steps=steps,
                seconds=secs, layers_per_s=P * steps / secs, ms_per_step=secs / steps * 1e3,
                layer_hbm_frac=(total_bytes / secs) / (HBM_PEAK_GBS * 1e9),
                state_gb=st.nbytes / 1e9, state_bytes=st.nbytes, expect_cut=ex, norm2=n2,
@@ -343,8 +343,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.dtype == "c128" else "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: N=30 P=5 C=13 G(30,0.5) weights U(0,1], "
-                               f"{args.dtype}, {args.schedule} schedule, one P=5 evaluation per step",
+        "config": {"workload": f"{args.workload}: {main_res['desc']}, "
+                               f"{args.dtype}, {args.schedule} schedule, "
+                               f"one P={main_res['P']} evaluation per step",
                    "N": main_res["N"], "P": main_res["P"], "C": main_res["C"],
                    "schedule": args.schedule, "parallelism": "1 GPU"},
         "layer_hbm_frac": main_res["layer_hbm_frac"],

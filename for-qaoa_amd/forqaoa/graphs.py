@@ -78,7 +78,7 @@ def total_weight(edges):
 def workload(name):
     """Named instances of the benchmark configurations.
 
-    Returns dict(N, P, C, edges, gammas, betas, seed, dtype_hint)."""
+    Returns dict(N, P, C, edges, gammas, betas, seed, name, desc)."""
     table = {
         # N=20, P=5, C=12, G(20, 0.5), weights in (0, 1]
         "n20_gnp05": dict(N=20, P=5, C=12, seed=BASE_SEED + 1, gen=("gnp", 0.5, "uniform")),
@@ -100,5 +100,8 @@ def workload(name):
     else:
         edges = regular(a, spec["N"], spec["seed"], wk)
     g, b = angles(spec["P"], spec["seed"])
-    spec.update(edges=edges, gammas=g, betas=b, name=name)
+    wdesc = {"uniform": "weights U(0,1]", "int16": "integer weights 1..16", "unit": "unit weights"}[wk]
+    gdesc = f"G({spec['N']},{a})" if kind == "gnp" else f"{a}-regular"
+    spec.update(edges=edges, gammas=g, betas=b, name=name,
+                desc=f"N={spec['N']} P={spec['P']} C={spec['C']} {gdesc} {wdesc}")
     return spec
