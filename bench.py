@@ -6,10 +6,12 @@ MI355X, with the achieved HBM bandwidth of the dominant kernel.
 
 Workload (value): N=30, P=5 weighted Max-Cut on the seeded G(30, 0.5) graph
 with weights uniform in (0, 1], per-layer angles uniform in [0, pi),
-complex128 amplitudes, the fused schedule (cost phase fused into the first
-tile pass, RX on the high qubits through strided LDS tiles).  One step is one
-full P=5 evaluation: the launch-control layer through layer 5, inputs and
-state resident in HBM.  value = 5 * steps / seconds.
+complex128 amplitudes, the fused schedule (cost phase fused into a tile
+pass, RX on the high qubits through strided LDS tiles, layers pipelined: the
+pass that ends layer p also applies the cost phase and RX of layer p+1 on its
+qubits, 11 tile passes for P=5).  One step is one full P=5 evaluation: the
+launch-control layer through layer 5, inputs and state resident in HBM.
+value = 5 * steps / seconds.
 
 Also reported (extra): the same evaluation on the 3-regular graph, in
 complex64, with the reference schedule (cost, RX chunks, transpositions), and

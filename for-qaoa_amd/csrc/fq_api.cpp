@@ -898,10 +898,15 @@ static int pass_args(int N, int dtype, const std::vector<int> &tile, const std::
 }
 
 // One tile pass with RX on gate positions (and optionally the cost phase).
+// pre_beta (with cost, not first, not a mirror pass): RX(*pre_beta) on the
+// same positions before the cost phase -- the pass that ends one layer and
+// opens the next.
 static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vector<int> &gates,
                      double beta, fq_graph *g, double gamma, bool cost, bool first, int kind,
-                     int z2 = Z2_OFF)
+                     int z2 = Z2_OFF, const double *pre_beta = nullptr)
 {
+    if (pre_beta && (!cost || first || z2 >= Z2_MIRROR))
+        return set_err(FQ_ERR_ARG, "internal: a layer-boundary pass needs the cost phase of a later layer");
     TileArgs a;
     Geom geom;
     if (int rc = pass_args(st->N, st->dtype, tile, gates, cost, z2, a, geom))
@@ -911,6 +916,11 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     sincos(beta / 2, &s, &c);
     a.rx_c = c;
     a.rx_s = s;
+    if (pre_beta) {
+        sincos(*pre_beta / 2, &s, &c);
+        a.pre_c = c;
+        a.pre_s = s;
+    }
     if (cost) {
         if (int rc = get_plan(g, geom, &a.plan, st->stream))
             return rc;
@@ -924,6 +934,8 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     return timed(st, kind, bytes, [&] {
         if (z2 >= Z2_MIRROR)
             return launch_tile_mirror(st->dtype, a_log, z2 == Z2_EXPAND, st->sv, a, st->stream);
+        if (pre_beta)
+            return launch_tile_pre(st->dtype, a_log, st->sv, a, st->d_phase, st->stream);
         return launch_tile(st->dtype, a_log, cost, first, true, st->sv, a, cost ? st->d_phase : nullptr,
                            st->stream);
     });
@@ -983,7 +995,7 @@ static bool carrier_gates()
 // of qubits per pass (N=30 complex128: strided passes 8.0 and 7.2 ms -> 6.8
 // and 7.0 ms; N=34 complex64: 65 and 98 ms -> 57 and 81 ms).  A pass that
 // would need more bit runs than the tile kernel takes gets larger blocks.
-static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
+static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype, int boundary = -1)
 {
     std::vector<FusedPass> ps;
     Qmax = std::min(Qmax, N);
@@ -1017,15 +1029,19 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
             const int a_log = tile_a_log(dtype, Qmax);
             // lane order: the 64 lanes of a wave are the tile's first six
             // positions; past the carriers they take the pass's lowest new
-            // qubits whose byte-address bit is outside [19, 22] (one load
-            // instruction whose lanes differ only in those bits measured up
+            // qubits whose byte-address bit is outside [19, hi] (one load
+            // instruction whose lanes differ only in bits 19..22 measured up
             // to 1.6x slower), else the highest; never N-1, which stays the
-            // top register bit for the half-state mirror pass
+            // top register bit for the half-state mirror pass. hi = 22 for
+            // states under 32 GiB; from 32 GiB on hi = 28 (lane bits 23..28
+            // measured 2-8% slower per layer at N=31..33 complex128 and N=34
+            // complex64, while at N=29 complex128 the window to 28 cost 1.5%)
             const int lanes = std::max(0, std::min(6, Qmax - a_log) - c);
             const int elem_log = dtype == FQ_C128 ? 4 : 3;
+            const int win_hi = N + elem_log >= 35 ? 28 : 22;
             std::vector<int> lane_q;
             for (int q : nw)
-                if ((int)lane_q.size() < lanes && q != N - 1 && (q + elem_log < 19 || q + elem_log > 22))
+                if ((int)lane_q.size() < lanes && q != N - 1 && (q + elem_log < 19 || q + elem_log > win_hi))
                     lane_q.push_back(q);
             for (int i = (int)nw.size() - 1; i >= 0 && (int)lane_q.size() < lanes; --i)
                 if (nw[i] != N - 1 && std::find(lane_q.begin(), lane_q.end(), nw[i]) == lane_q.end())
@@ -1050,11 +1066,15 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype)
                 // the cost phase of pass 0): give each to the strided pass
                 // with the fewest gates, which takes LDS phases and VALU work
                 // off the cost pass
+                // -- not to the pass given as boundary (pipelined layers run
+                // its RX twice per launch), unless no other pass carries q
                 for (int q = 0; q < Qmax; ++q) {
                     int best = -1;
-                    for (int k2 = 0; k2 < npass; ++k2)
-                        if (q < Qmax - want[k2] && (best < 0 || more[k2].gates.size() < more[best].gates.size()))
-                            best = k2;
+                    for (int pass = 0; pass < 2 && best < 0; ++pass)
+                        for (int k2 = 0; k2 < npass; ++k2)
+                            if (q < Qmax - want[k2] && (pass == 1 || k2 + 1 != boundary) &&
+                                (best < 0 || more[k2].gates.size() < more[best].gates.size()))
+                                best = k2;
                     if (best < 0)
                         continue;
                     more[best].gates.push_back(q);
@@ -1193,11 +1213,100 @@ extern "C" int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_sw
 
 // ---- schedules -----------------------------------------------------------------------
 
-// Tile passes of one fused layer: pass 1 owns the lowest Qmax qubits
-// (contiguous tile, cost phase fused), the others are strided.
-static std::vector<FusedPass> fused_passes(int N, int dtype)
+
+// Whether the layers of the fused and half-state schedules are pipelined
+// (run_layers): by default for complex128 only (measured on MI355X at N=30:
+// complex128 50.0 -> 62.5 layers/s; complex64 94 -> 91, its boundary pass
+// spills registers).  FQ_LAYER_PIPE (environment, read per call) = 1 or 0
+// forces it on or off.
+static bool layer_pipe(int dtype)
 {
-    return cover_passes(N, pass_tile_q(dtype), dtype);
+    const char *e = getenv("FQ_LAYER_PIPE");
+    return e ? atoi(e) != 0 : dtype == FQ_C128;
+}
+
+// The passes of one layer of the fused (z2 false) or half-state schedule and,
+// when the layers are pipelined, the pass besides pass 0 that ends every
+// other layer (end1 < 0: layer order).  end1 is the last pass of the fused
+// schedule and pass 1 of the half-state one (its last pass is the mirror
+// pass, which has no cost phase); the carried low qubits' RX goes to the
+// other passes.  Measured at N=30 complex128 against ends {0, 1} with the
+// carried RX on the fewest-gate pass: fused 62.5 -> 65.0 layers/s,
+// half state 120.9 -> 123.8.
+struct LayerPlan {
+    std::vector<FusedPass> ps;
+    int end1;
+};
+
+static LayerPlan layer_plan(int N, int dtype, bool z2)
+{
+    const int Qmax = z2 ? std::min(pass_tile_q(dtype), N - 1) : pass_tile_q(dtype);
+    LayerPlan lp{cover_passes(N, Qmax, dtype), -1};
+    const int m = (int)lp.ps.size();
+    if (layer_pipe(dtype) && m >= (z2 ? 3 : 2)) {
+        lp.end1 = z2 ? 1 : m - 1;
+        if (m >= 3)
+            lp.ps = cover_passes(N, Qmax, dtype, lp.end1);
+    }
+    return lp;
+}
+
+// The tile passes of P layers (pass 0 the low contiguous tile; z2: half-state
+// passes, the last one the mirror pass).  In layer order, layer p runs passes
+// 0..m-1 with its cost phase in pass 0.  Pipelined: the cost phase is
+// diagonal, so it can run in any pass; the pass that ends layer p applies
+// RX(beta_p), the phase of gamma_{p+1} and RX(beta_{p+1}) on its qubits,
+// which opens layer p+1.  Layers end alternately on end1 and pass 0, so every
+// layer after the first runs m-1 passes: 1 + P(m-1) passes in all instead of
+// Pm.  Every RX of a layer still follows its cost phase and precedes the next
+// one, and RX gates on distinct qubits commute.
+static int run_layers(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas, bool z2)
+{
+    const LayerPlan lp = layer_plan(st->N, st->dtype, z2);
+    const std::vector<FusedPass> &ps = lp.ps;
+    const int m = (int)ps.size();
+    auto mode = [&](int i, bool last) {
+        if (!z2)
+            return (int)Z2_OFF;
+        return i + 1 < m ? (int)Z2_LOWER : last ? (int)Z2_EXPAND : (int)Z2_MIRROR;
+    };
+    auto plain = [&](int i, int p) {
+        return tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, 0.0, false, false, FQ_K_FUSED,
+                         mode(i, p + 1 == P));
+    };
+    if (lp.end1 < 0) {
+        for (int p = 0; p < P; ++p)
+            for (int i = 0; i < m; ++i) {
+                const int rc = i == 0 ? tile_pass(st, ps[0].tile, ps[0].gates, betas[p], g, gammas[p], true,
+                                                  p == 0, FQ_K_FUSED, mode(0, p + 1 == P))
+                                      : plain(i, p);
+                if (rc)
+                    return rc;
+            }
+        return FQ_OK;
+    }
+    for (int p = 0; p < P; ++p) {
+        const bool last = p + 1 == P;
+        const int e = p % 2 == 0 ? lp.end1 : 0;   // the pass that ends layer p (the other one opened it)
+        if (p == 0)
+            if (int rc = tile_pass(st, ps[0].tile, ps[0].gates, betas[0], g, gammas[0], true, true, FQ_K_FUSED,
+                                   mode(0, last)))
+                return rc;
+        if (last)
+            if (int rc = plain(e, p))
+                return rc;
+        for (int i = 1; i < m; ++i) {
+            if (i == lp.end1)
+                continue;
+            if (int rc = plain(i, p))
+                return rc;
+        }
+        if (!last)
+            if (int rc = tile_pass(st, ps[e].tile, ps[e].gates, betas[p + 1], g, gammas[p + 1], true, false,
+                                   FQ_K_FUSED, mode(e, false), &betas[p]))
+                return rc;
+    }
+    return FQ_OK;
 }
 
 extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
@@ -1245,30 +1354,11 @@ extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, co
         // half z_{N-1} = 0 and the other half is written once at the end.
         if (N < 3)
             return set_err(FQ_ERR_ARG, "the half-state schedule needs N >= 3");
-        const std::vector<FusedPass> ps = cover_passes(N, std::min(pass_tile_q(st->dtype), N - 1), st->dtype);
-        for (int p = 0; p < P; ++p) {
-            for (size_t i = 0; i < ps.size(); ++i) {
-                const bool cost = i == 0;
-                const int mode = i + 1 < ps.size() ? Z2_LOWER : p + 1 < P ? Z2_MIRROR : Z2_EXPAND;
-                if (int rc = tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, gammas[p], cost,
-                                       cost && p == 0, FQ_K_FUSED, mode))
-                    return rc;
-            }
-        }
-        return FQ_OK;
+        return run_layers(st, g, P, gammas, betas, true);
     }
     if (schedule != FQ_SCHED_FUSED)
         return set_err(FQ_ERR_ARG, "unknown schedule %d", schedule);
-    const std::vector<FusedPass> ps = fused_passes(N, st->dtype);
-    for (int p = 0; p < P; ++p) {
-        for (size_t i = 0; i < ps.size(); ++i) {
-            const bool cost = i == 0;
-            if (int rc = tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, gammas[p], cost,
-                                   cost && p == 0, FQ_K_FUSED))
-                return rc;
-        }
-    }
-    return FQ_OK;
+    return run_layers(st, g, P, gammas, betas, false);
 }
 
 extern "C" int fq_layer_passes(int N, int dtype, int schedule, int max_pass, int *npass, int *tile_q,
@@ -1277,10 +1367,8 @@ extern "C" int fq_layer_passes(int N, int dtype, int schedule, int max_pass, int
     if (N < 1 || N > 36 || (dtype != FQ_C128 && dtype != FQ_C64) || !npass || !tile_q || !tiles || !gates)
         return set_err(FQ_ERR_ARG, "bad N, dtype or null output");
     std::vector<FusedPass> ps;
-    if (schedule == FQ_SCHED_FUSED)
-        ps = fused_passes(N, dtype);
-    else if (schedule == FQ_SCHED_Z2 && N >= 3)
-        ps = cover_passes(N, std::min(pass_tile_q(dtype), N - 1), dtype);
+    if (schedule == FQ_SCHED_FUSED || (schedule == FQ_SCHED_Z2 && N >= 3))
+        ps = layer_plan(N, dtype, schedule == FQ_SCHED_Z2).ps;
     else
         return set_err(FQ_ERR_ARG, "schedule must be FQ_SCHED_FUSED or FQ_SCHED_Z2 (N >= 3)");
     if ((int)ps.size() > max_pass)

@@ -68,6 +68,7 @@ struct TileArgs {
     uint32_t grp_gates_w;    // grp_gates[0..4], 4 bits each
     uint32_t direct_store;   // last LDS phase holds no lane bit: store from its layout
     double rx_c, rx_s;       // cos(beta/2), sin(beta/2) (host-computed, src/rx.cu:121)
+    double pre_c, pre_s;     // layer-boundary pass: RX of the previous layer, applied first
     double gamma, init_amp;  // cost phase angle, launch-control amplitude
     int has_rr;              // any register-register edge in the plan
     PlanDev plan;
@@ -117,6 +118,9 @@ int device_cu_count();
 // launch status.
 hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, void *sv,
                        const TileArgs &a, const double2 *phase_tab, hipStream_t stream);
+// Layer-boundary pass: RX(pre) on the pass's qubits, cost phase, RX(rx).
+hipError_t launch_tile_pre(int dtype, int a_log, void *sv, const TileArgs &a, const double2 *ptab,
+                           hipStream_t stream);
 // Mirror pass of a Z2-reduced state: the top register bit of the virtual tile
 // selects the complementary tile (complemented element offsets) and RX on it
 // is the butterfly of the global spin flip.  expand: also write the upper

@@ -129,7 +129,64 @@ __device__ __forceinline__ uint64_t tile_at(const TileArgs &a, uint64_t ntiles, 
 // LDS phases, the next tile's amplitudes are already loading into registers.
 // QT > 0 fixes the tile width at compile time (the fused passes' full-width
 // tiles); QT == 0 reads it from the arguments.
-template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0>
+// RX on the gate positions of a tile held in the natural layout (register j
+// of thread t = tile element t + j * 2^nthr), returned in the same layout:
+// register bits in registers, thread bits through the LDS remapping phases.
+// The caller's previous LDS access was a read of its own natural slots (or
+// none), so the first write here needs no barrier.
+template <typename T, typename R, int A_LOG>
+__device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &a, const int *etp, int nat,
+                                            int nthr, R c, R s)
+{
+    constexpr int A = 1 << A_LOG;
+    extern __shared__ __align__(16) unsigned char fq_smem[];
+    rx_registers<T, R, A_LOG>(v, a.gate_mask >> nthr, c, s);
+    if (a.ngroups == 0)
+        return;
+    T *lds = reinterpret_cast<T *>(fq_smem);
+#pragma unroll
+    for (int j = 0; j < A; ++j)
+        lds[nat + lds_pad(j << nthr)] = v[j];
+    lds_barrier();
+#pragma unroll 1
+    for (int g = 0; g < a.ngroups; ++g) {
+        const int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
+        const unsigned gm = group_mask(a, g);
+        int pr[A_LOG > 0 ? A_LOG : 1];
+        unsigned g2 = gm;
+#pragma unroll
+        for (int r = 0; r < A_LOG; ++r) {
+            pr[r] = __builtin_ctz(g2);
+            g2 &= g2 - 1u;
+        }
+        int ej[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j) {
+            int e = 0;
+#pragma unroll
+            for (int r = 0; r < A_LOG; ++r)
+                if ((j >> r) & 1)
+                    e |= 1 << pr[r];
+            ej[j] = lds_pad(e);
+        }
+#pragma unroll
+        for (int j = 0; j < A; ++j)
+            v[j] = lds[eg + ej[j]];
+        rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
+#pragma unroll
+        for (int j = 0; j < A; ++j)
+            lds[eg + ej[j]] = v[j];
+        lds_barrier();
+    }
+#pragma unroll
+    for (int j = 0; j < A; ++j)
+        v[j] = lds[nat + lds_pad(j << nthr)];
+}
+
+// PRE: the pass that ends one layer and opens the next applies RX of the
+// previous layer (pre_c, pre_s) on the same gate positions before the cost
+// phase (COST, not FIRST, MIRROR 0 only).
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false>
 __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a,
                                                             const double2 *__restrict__ ptab)
 {
@@ -246,6 +303,9 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     }
     while (tile < ntiles) {
         const uint64_t next = tile_at(a, ntiles, ++kt);
+
+        if (PRE)
+            mix_natural<T, R, A_LOG>(v, a, etp, nat, nthr, (R)a.pre_c, (R)a.pre_s);
 
         // Cost phase first (few registers live), then the prefetch.
         // exp(i*gamma*s/2), s = 2*Cut - W, of register j is the product of the
@@ -943,14 +1003,14 @@ int occupancy_per_cu(const void *fn, int threads, size_t lds)
     return per_cu;
 }
 
-template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES>
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, bool PRE = false>
 static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab, hipStream_t stream)
 {
     typedef typename CT<R>::T T;
     const unsigned threads = 1u << a.nthr;
     const uint64_t tiles = run_tiles(a);
     const size_t lds = GATES && a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
-    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES>);
+    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE>);
     if (lds > 65536) {
         if (hipError_t e = allow_large_lds(fn))
             return e;
@@ -963,7 +1023,7 @@ static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab
     TileArgs la = a;
     if (la.order == 1 && grid % 8 != 0)
         la.order = 0;
-    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES>), dim3((unsigned)grid), dim3(threads),
+    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE>), dim3((unsigned)grid), dim3(threads),
                        lds, stream, reinterpret_cast<T *>(sv), la, ptab);
     return hipGetLastError();
 }
@@ -1095,6 +1155,40 @@ hipError_t launch_tile(int dtype, int a_log, bool cost, bool first, bool gates, 
     if (dtype == 0)
         return launch_tile_r<double>(a_log, cost, first, gates, sv, a, ptab, stream);
     return launch_tile_r<float>(a_log, cost, first, gates, sv, a, ptab, stream);
+}
+
+template <typename R>
+static hipError_t launch_pre_r(int a_log, void *sv, const TileArgs &a, const double2 *ptab, hipStream_t stream)
+{
+    if constexpr (sizeof(R) == 8) {
+        if (a_log == 3 && a.Q == 13)
+            return launch_tile_t<R, 3, 13, true, false, true, true>(sv, a, ptab, stream);
+        if (a_log == 3 && a.Q == 12)
+            return launch_tile_t<R, 3, 12, true, false, true, true>(sv, a, ptab, stream);
+    } else {
+        if (a_log == 4 && a.Q == 14)
+            return launch_tile_t<R, 4, 14, true, false, true, true>(sv, a, ptab, stream);
+        if (a_log == 4 && a.Q == 13)
+            return launch_tile_t<R, 4, 13, true, false, true, true>(sv, a, ptab, stream);
+    }
+    switch (a_log) {
+        case 1: return launch_tile_t<R, 1, 0, true, false, true, true>(sv, a, ptab, stream);
+        case 2: return launch_tile_t<R, 2, 0, true, false, true, true>(sv, a, ptab, stream);
+        case 3: return launch_tile_t<R, 3, 0, true, false, true, true>(sv, a, ptab, stream);
+        case 4:
+            if constexpr (sizeof(R) == 4)
+                return launch_tile_t<R, 4, 0, true, false, true, true>(sv, a, ptab, stream);
+            return hipErrorInvalidValue;
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tile_pre(int dtype, int a_log, void *sv, const TileArgs &a, const double2 *ptab,
+                           hipStream_t stream)
+{
+    if (dtype == 0)
+        return launch_pre_r<double>(a_log, sv, a, ptab, stream);
+    return launch_pre_r<float>(a_log, sv, a, ptab, stream);
 }
 
 // Streaming probe (fq_stream_probe): mode 0 scales n 8-byte elements in

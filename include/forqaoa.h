@@ -128,7 +128,12 @@ int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_swap);
 /* The driver loop (src/Weighted.cpp:97-137) for P layers with per-layer
  * angles gammas[p], betas[p].  FQ_SCHED_REFERENCE also relabels g after each
  * transposition exactly as the driver does (g ends in the state's physical
- * labels); FQ_SCHED_FUSED and FQ_SCHED_Z2 leave the layout and g unchanged. */
+ * labels); FQ_SCHED_FUSED and FQ_SCHED_Z2 leave the layout and g unchanged.
+ * With two or more passes per layer (three for FQ_SCHED_Z2) and complex128,
+ * the fused schedules pipeline the layers: the pass that ends layer p also
+ * applies the cost phase and RX of layer p+1 on its qubits, so every layer
+ * after the first runs one pass fewer (environment FQ_LAYER_PIPE=0/1 forces
+ * it off/on for either precision). */
 int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
            int schedule);
 
@@ -212,7 +217,9 @@ int fq_slab_copy_out(fq_slab *sb, int slot, void *host, size_t bytes);
  * *npass passes of *tile_q positions each; pass p's tile positions (thread
  * bits first, the last ones register bits) in tiles[16*p .. 16*p+15] (-1
  * padded) and its RX targets as the bit mask gates[p].  Pass 0 applies the
- * cost phase.  max_pass bounds the output arrays. */
+ * cost phase (when fq_run pipelines the layers, passes 0 and 1 alternately
+ * end a layer and apply the next layer's cost phase).  max_pass bounds the
+ * output arrays. */
 int fq_layer_passes(int N, int dtype, int schedule, int max_pass, int *npass, int *tile_q, int *tiles,
                     uint64_t *gates);
 

@@ -188,3 +188,17 @@ def test_layer_passes_n30_c128():
     assert len(ps) == 3
     assert ps[0][1] == list(range(5, 13))       # carried qubits 0..4 moved to the strided passes
     assert sorted(ps[1][1] + ps[2][1]) == [0, 1, 2, 3, 4] + list(range(13, 30))
+
+
+@pytest.mark.parametrize("N,dtype,lanes", [
+    (30, "c128", [[13, 14], [4, 20]]),   # 17 GiB state: lane bytes bits outside [19, 22]
+    (29, "c128", [[4, 13], [4, 19]]),
+    (32, "c128", [[14, 25], [26, 27]]),  # from 32 GiB on: outside [19, 28]
+    (34, "c64", [[14, 15], [28, 29]]),
+])
+def test_layer_passes_wave_lanes(N, dtype, lanes):
+    # tile positions 0..5 are the 64 lanes of a wave; past the carriers they
+    # hold the lowest new qubits whose byte-address bit avoids the window
+    from forqaoa import layer_passes
+    ps = layer_passes(N, dtype, "fused")
+    assert [t[4:6] for t, _ in ps[1:]] == lanes
