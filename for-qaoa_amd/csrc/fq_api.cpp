@@ -1215,14 +1215,13 @@ extern "C" int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_sw
 
 
 // Whether the layers of the fused and half-state schedules are pipelined
-// (run_layers): by default for complex128 only (measured on MI355X at N=30:
-// complex128 50.0 -> 62.5 layers/s; complex64 94 -> 91, its boundary pass
-// spills registers).  FQ_LAYER_PIPE (environment, read per call) = 1 or 0
-// forces it on or off.
-static bool layer_pipe(int dtype)
+// (run_layers): by default (measured on MI355X, N=30 fused: complex128
+// 50.0 -> 65.0 layers/s, complex64 94 -> 110); FQ_LAYER_PIPE=0 (environment,
+// read per call) runs them in layer order.
+static bool layer_pipe()
 {
     const char *e = getenv("FQ_LAYER_PIPE");
-    return e ? atoi(e) != 0 : dtype == FQ_C128;
+    return e ? atoi(e) != 0 : true;
 }
 
 // The passes of one layer of the fused (z2 false) or half-state schedule and,
@@ -1243,7 +1242,7 @@ static LayerPlan layer_plan(int N, int dtype, bool z2)
     const int Qmax = z2 ? std::min(pass_tile_q(dtype), N - 1) : pass_tile_q(dtype);
     LayerPlan lp{cover_passes(N, Qmax, dtype), -1};
     const int m = (int)lp.ps.size();
-    if (layer_pipe(dtype) && m >= (z2 ? 3 : 2)) {
+    if (layer_pipe() && m >= (z2 ? 3 : 2)) {
         lp.end1 = z2 ? 1 : m - 1;
         if (m >= 3)
             lp.ps = cover_passes(N, Qmax, dtype, lp.end1);

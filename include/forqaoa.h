@@ -129,11 +129,10 @@ int fq_swap_conti(fq_state *st, int swap_out, int swap_in, int num_swap);
  * angles gammas[p], betas[p].  FQ_SCHED_REFERENCE also relabels g after each
  * transposition exactly as the driver does (g ends in the state's physical
  * labels); FQ_SCHED_FUSED and FQ_SCHED_Z2 leave the layout and g unchanged.
- * With two or more passes per layer (three for FQ_SCHED_Z2) and complex128,
- * the fused schedules pipeline the layers: the pass that ends layer p also
- * applies the cost phase and RX of layer p+1 on its qubits, so every layer
- * after the first runs one pass fewer (environment FQ_LAYER_PIPE=0/1 forces
- * it off/on for either precision). */
+ * With two or more passes per layer (three for FQ_SCHED_Z2) the fused
+ * schedules pipeline the layers: the pass that ends layer p also applies the
+ * cost phase and RX of layer p+1 on its qubits, so every layer after the
+ * first runs one pass fewer (environment FQ_LAYER_PIPE=0 turns it off). */
 int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas,
            int schedule);
 

@@ -2,8 +2,8 @@
 
 The pass that ends layer p applies RX(beta_p), the cost phase of layer p+1 and
 RX(beta_{p+1}) on its qubits, so a run of P layers with m passes per layer
-launches 1 + P(m-1) tile passes instead of Pm (complex128 by default;
-FQ_LAYER_PIPE=1 forces it for complex64, =0 turns it off).  Checked here:
+launches 1 + P(m-1) tile passes instead of Pm (FQ_LAYER_PIPE=0 turns it
+off).  Checked here:
 
 * against the oracle's state vector (an independent layer-by-layer path) for
   qubit counts with 2 and 3 passes per layer, odd and even depths (layers end
@@ -66,7 +66,7 @@ CASES = [
     # 3 passes per layer (fused from N=24 complex128; half state from N=24)
     (24, "c128", "fused", 3), (24, "c128", "z2", 2), (24, "c128", "z2", 3), (25, "c128", "z2", 4),
     (24, "c128", "fused", 4),
-    # complex64 forced on
+    # complex64
     (15, "c64", "fused", 3), (24, "c64", "fused", 2), (25, "c64", "z2", 3), (26, "c64", "z2", 4),
 ]
 
@@ -99,11 +99,8 @@ def test_pipelined_matches_layer_order(N, dtype, schedule, P, pipe_env):
     assert err < TOL[dtype]
 
 
-def test_default_is_complex128_only(pipe_env):
-    # default: complex128 pipelined, complex64 in layer order
-    _, l128, *_ = _run(24, "c128", "fused", 3, 1, None)
-    _, l64, *_ = _run(24, "c64", "fused", 3, 1, None)
-    m128 = len(layer_passes(24, "c128", "fused"))
-    m64 = len(layer_passes(24, "c64", "fused"))
-    assert l128 == 1 + 3 * (m128 - 1)
-    assert l64 == 3 * m64
+def test_pipelined_by_default(pipe_env):
+    for dtype in ("c128", "c64"):
+        _, launches, *_ = _run(24, dtype, "fused", 3, 1, None)
+        m = len(layer_passes(24, dtype, "fused"))
+        assert launches == 1 + 3 * (m - 1), dtype
