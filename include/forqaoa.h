@@ -215,9 +215,10 @@ int fq_slab_copy_out(fq_slab *sb, int slot, void *host, size_t bytes);
  * qubits (no counterpart in the reference; host-only, no device needed):
  * *npass passes of *tile_q positions each; pass p's tile positions (thread
  * bits first, the last ones register bits) in tiles[16*p .. 16*p+15] (-1
- * padded) and its RX targets as the bit mask gates[p].  Pass 0 applies the
- * cost phase (when fq_run pipelines the layers, passes 0 and 1 alternately
- * end a layer and apply the next layer's cost phase).  max_pass bounds the
+ * padded) and its RX targets as the bit mask gates[p].  In layer order pass
+ * 0 applies every layer's cost phase; with pipelined layers it applies layer
+ * 0's, and pass 0 and the last pass (FQ_SCHED_FUSED) or pass 1 (FQ_SCHED_Z2)
+ * alternately end a layer and apply the next one's.  max_pass bounds the
  * output arrays. */
 int fq_layer_passes(int N, int dtype, int schedule, int max_pass, int *npass, int *tile_q, int *tiles,
                     uint64_t *gates);
