@@ -124,11 +124,6 @@ __device__ __forceinline__ uint64_t tile_at(const TileArgs &a, uint64_t ntiles, 
     return k * G + b;
 }
 
-// Persistent tile pass.  Each work-group walks tiles blockIdx.x,
-// blockIdx.x + gridDim.x, ...; while a tile goes through its cost phase and
-// LDS phases, the next tile's amplitudes are already loading into registers.
-// QT > 0 fixes the tile width at compile time (the fused passes' full-width
-// tiles); QT == 0 reads it from the arguments.
 // RX on the gate positions of a tile held in the natural layout (register j
 // of thread t = tile element t + j * 2^nthr), returned in the same layout:
 // register bits in registers, thread bits through the LDS remapping phases.
@@ -183,6 +178,11 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
         v[j] = lds[nat + lds_pad(j << nthr)];
 }
 
+// Persistent tile pass.  Each work-group walks tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ...; while a tile goes through its cost phase and
+// LDS phases, the next tile's amplitudes are already loading into registers.
+// QT > 0 fixes the tile width at compile time (the fused passes' full-width
+// tiles); QT == 0 reads it from the arguments.
 // PRE: the pass that ends one layer and opens the next applies RX of the
 // previous layer (pre_c, pre_s) on the same gate positions before the cost
 // phase (COST, not FIRST, MIRROR 0 only).
