@@ -202,3 +202,32 @@ def test_layer_passes_wave_lanes(N, dtype, lanes):
     from forqaoa import layer_passes
     ps = layer_passes(N, dtype, "fused")
     assert [t[4:6] for t, _ in ps[1:]] == lanes
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("schedule", ["fused", "z2"])
+def test_boundary_pass_gets_no_carried_rx(dtype, schedule):
+    # pipelined layers end alternately on pass 0 and on the last pass (fused)
+    # or pass 1 (half state); with 3 or more passes per layer the carried low
+    # qubits' RX goes to the other strided passes, which run it once per layer
+    from forqaoa import layer_passes
+    qmax = 13 if dtype == "c128" else 14
+    for N in range(qmax + 2, 35):
+        ps = layer_passes(N, dtype, schedule)
+        if len(ps) < 3:
+            continue
+        q0 = len(ps[0][0])
+        boundary = len(ps) - 1 if schedule == "fused" else 1
+
+        def carriers(tile):
+            c = 0
+            while c < len(tile) and tile[c] == c:
+                c += 1
+            return c
+
+        plain = [k for k in range(1, len(ps)) if k != boundary]
+        # a low qubit's RX sits in the boundary pass only if no plain pass carries it
+        for q in ps[boundary][1]:
+            if q < q0:
+                assert all(q >= carriers(ps[k][0]) for k in plain), (N, q, ps)
+        assert any(q < q0 for k in plain for q in ps[k][1]), (N, ps)
