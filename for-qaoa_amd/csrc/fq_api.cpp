@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -78,17 +79,113 @@ extern "C" int fq_memory_in_use(size_t *bytes)
 }
 extern "C" int fq_version(void) { return 100; }
 
+// ---- RX coefficients ---------------------------------------------------------------
+
+void fq::rx_coeffs(double beta, int dtype, double *c, double *s)
+{
+    double sn, cs;
+    sincos(beta / 2, &sn, &cs);
+    if (dtype != FQ_C64) {
+        *c = cs;
+        *s = sn;
+        return;
+    }
+    // Search FP32 pairs within 3 ulps of the rounded |cos|, |sin| for the
+    // smallest |c^2 + s^2 - 1| + |angle error| (both exact or nearly so in FP64:
+    // a product of two FP32 values is exact).  Working on magnitudes and
+    // restoring the signs afterwards makes RX(-beta) the exact inverse-transpose
+    // partner of RX(beta), as the adjoint's uncompute needs.
+    const double ac = std::fabs(cs), as = std::fabs(sn);
+    const double half = std::atan2(as, ac);
+    const float c0 = (float)ac, s0 = (float)as;
+    float cb = c0, sb = s0;
+    double best = HUGE_VAL;
+    for (int i = -3; i <= 3; ++i) {
+        float ci = c0;
+        for (int k = 0; k < std::abs(i); ++k)
+            ci = std::nextafterf(ci, i < 0 ? 0.0f : 2.0f);
+        if (ci < 0.0f || ci > 1.0f)
+            continue;
+        for (int j = -3; j <= 3; ++j) {
+            float sj = s0;
+            for (int k = 0; k < std::abs(j); ++k)
+                sj = std::nextafterf(sj, j < 0 ? 0.0f : 2.0f);
+            if (sj < 0.0f || sj > 1.0f)
+                continue;
+            const double dn = (double)ci * ci + (double)sj * sj - 1.0;
+            const double da = std::atan2((double)sj, (double)ci) - half;
+            const double score = std::fabs(dn) + std::fabs(da);
+            if (score < best) {
+                best = score;
+                cb = ci;
+                sb = sj;
+            }
+        }
+    }
+    *c = std::copysign((double)cb, cs);
+    *s = std::copysign((double)sb, sn);
+}
+
+// RX(beta) as applied with rx_coeffs is sqrt(c^2 + s^2) times a unitary, a
+// global factor per gate.  A P-layer evaluation gives every qubit one RX per
+// layer, so its state is prod_p (c_p^2 + s_p^2)^(N/2) times the state of a
+// unitary circuit; scaling the launch-control amplitude by the inverse removes
+// that factor exactly (1 for complex128, where c^2 + s^2 - 1 is ~1e-16).
+static double rx_norm_scale(int N, int P, const double *betas, int dtype)
+{
+    if (dtype != FQ_C64)
+        return 1.0;
+    double sc = 1.0;
+    for (int p = 0; p < P; ++p) {
+        double c, s;
+        rx_coeffs(betas[p], dtype, &c, &s);
+        sc *= std::pow(c * c + s * s, -0.5 * N);
+    }
+    return sc;
+}
+
+// ---- streams the library created -----------------------------------------------
+//
+// States and slabs run on streams the library creates (a state may be moved to
+// a caller's stream instead).  A cached cost plan that is replaced must first
+// be released by every stream it was handed to; the library can order that on
+// the device only for its own live streams (a destroyed one finished its work
+// when it was destroyed), so it keeps a registry of them.
+
+static std::mutex g_streams_mu;
+static std::set<hipStream_t> g_own_streams;
+
+static void own_stream(hipStream_t s, bool live)
+{
+    std::lock_guard<std::mutex> lk(g_streams_mu);
+    if (live)
+        g_own_streams.insert(s);
+    else
+        g_own_streams.erase(s);
+}
+
+static bool is_own_stream(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(g_streams_mu);
+    return g_own_streams.count(s) != 0;
+}
+
 // ---- handles -------------------------------------------------------------------
 
 struct PlanEntry {
     void *dmem = nullptr;
+    size_t cap = 0;               // bytes of dmem and staging
     PlanDev dev{};
     void *staging = nullptr;      // pinned copy of the blob (asynchronous upload)
     hipEvent_t ready = nullptr;   // upload done
     hipStream_t stream = nullptr; // stream the upload was queued on
+    std::vector<hipStream_t> users; // streams the plan was handed to
+    bool foreign = false;         // handed to a stream the library did not create
+    uint64_t last_use = 0;        // fq_graph::use_clock at the last lookup
 };
 
-// Releases a plan; the caller has synchronized the device.
+// Releases a plan's memory; the caller has synchronized the device (hipFree
+// waits for the whole device anyway).
 static void plan_free(PlanEntry &e)
 {
     hipFree(e.dmem);
@@ -98,14 +195,18 @@ static void plan_free(PlanEntry &e)
     e = PlanEntry();
 }
 
+// Plans cached per graph; beyond this the least recently used plan's buffers
+// are reused for the next one.
+#define FQ_PLAN_CACHE 64
+
 struct fq_graph {
     int N = 0;
     std::vector<double> W;  // N*N, W[min*N+max], current labels
+    // key: tile geometry + the bytes of W, so a hit is an exact content match
     std::map<std::string, PlanEntry> plans;
     uint64_t version = 1;       // bumped by every change of W
-    uint64_t hashed_version = 0;
-    uint64_t w_hash = 0;        // content hash of W at hashed_version
-    uint64_t plan_epoch = 0;    // bumped when the plan cache is released
+    uint64_t plan_epoch = 0;    // bumped when a cached plan is dropped or reused
+    uint64_t use_clock = 0;
 };
 
 struct TimingRec {
@@ -272,14 +373,12 @@ static int geom_runs(const Geom &g)
 
 // Tile order of the persistent passes (tile_at): 2, one contiguous range of
 // tiles per work-group (measured fastest on MI355X for the contiguous and
-// the strided passes); FQ_TILE_ORDER (environment) selects another.
+// the strided passes); FQ_TILE_ORDER (environment, read per call) selects
+// another.
 static uint32_t tile_order()
 {
-    static const uint32_t order = [] {
-        const char *e = getenv("FQ_TILE_ORDER");
-        return (uint32_t)(e ? std::max(0, std::min(3, atoi(e))) : 2);
-    }();
-    return order;
+    const char *e = getenv("FQ_TILE_ORDER");
+    return (uint32_t)(e ? std::max(0, std::min(3, atoi(e))) : 2);
 }
 
 static void geom_to_args(const Geom &g, TileArgs &a)
@@ -358,35 +457,32 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
 
 // ---- cost plans ------------------------------------------------------------------
 
-static uint64_t hash_bytes(const void *p, size_t n)
-{
-    const unsigned char *b = (const unsigned char *)p;
-    uint64_t h = 1469598103934665603ull;
-    for (size_t i = 0; i < n; ++i) {
-        h ^= b[i];
-        h *= 1099511628211ull;
-    }
-    return h;
-}
-
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // Device cut plan of graph G for tile geometry g, cached per (geometry, W
 // content).  A new plan is uploaded asynchronously on `stream`; a cached plan
 // whose upload was queued on another stream makes `stream` wait for it.
+// With FQ_PLAN_CACHE plans cached, the least recently used one whose buffers
+// are large enough is replaced: `stream` first waits (on the device, through
+// an event) for the work queued so far on every stream that plan was handed
+// to, so no launch still reading it sees the new contents, and the host
+// blocks on nothing but that plan's own earlier upload.
 static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream)
 {
     const int N = G->N;
-    if (G->hashed_version != G->version) {
-        G->w_hash = hash_bytes(G->W.data(), G->W.size() * sizeof(double));
-        G->hashed_version = G->version;
-    }
-    const uint64_t wh = G->w_hash;
     std::string key = geom_key(g);
-    key.append(reinterpret_cast<const char *>(&wh), sizeof(wh));
+    key.append(reinterpret_cast<const char *>(G->W.data()), G->W.size() * sizeof(double));
+    auto note_user = [&](PlanEntry &e) {
+        e.last_use = ++G->use_clock;
+        if (std::find(e.users.begin(), e.users.end(), stream) == e.users.end()) {
+            e.users.push_back(stream);
+            e.foreign |= !is_own_stream(stream);
+        }
+    };
     auto it = G->plans.find(key);
     if (it != G->plans.end()) {
         PlanEntry &e = it->second;
+        note_user(e);
         if (e.ready && e.stream != stream) {
             const hipError_t q = hipEventQuery(e.ready);
             if (q == hipErrorNotReady) {
@@ -398,13 +494,6 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream
         }
         *out = e.dev;
         return FQ_OK;
-    }
-    if (G->plans.size() >= 64) {
-        FQ_HIP(hipDeviceSynchronize());
-        for (auto &kv : G->plans)
-            plan_free(kv.second);
-        G->plans.clear();
-        G->plan_epoch++;
     }
     int ti[FQ_MAXN], hi[FQ_MAXN];
     for (int p = 0; p < N; ++p)
@@ -515,11 +604,53 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream
     put(o_hu, hh_u.data(), sizeof(int) * hh_u.size());
     put(o_hv, hh_v.data(), sizeof(int) * hh_v.size());
     PlanEntry pe;
-    if (hipMalloc(&pe.dmem, total) != hipSuccess || hipHostMalloc(&pe.staging, total, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&pe.ready, hipEventDisableTiming) != hipSuccess) {
-        hipGetLastError();
-        plan_free(pe);
-        return set_err(FQ_ERR_OOM, "plan allocation of %zu bytes failed", total);
+    bool reused = false;
+    if (G->plans.size() >= FQ_PLAN_CACHE) {
+        auto victim = G->plans.end();
+        for (auto v = G->plans.begin(); v != G->plans.end(); ++v)
+            if (v->second.cap >= total && (victim == G->plans.end() || v->second.last_use < victim->second.last_use))
+                victim = v;
+        if (victim == G->plans.end()) {
+            // every cached plan is smaller than this one: drop them all (rare;
+            // hipFree waits for the device in any case)
+            FQ_HIP(hipDeviceSynchronize());
+            for (auto &kv : G->plans)
+                plan_free(kv.second);
+            G->plans.clear();
+        } else {
+            pe = victim->second;
+            G->plans.erase(victim);
+            reused = true;
+            // the host rewrites staging: its previous upload must be done
+            FQ_HIP(hipEventSynchronize(pe.ready));
+            if (pe.foreign) {
+                // a caller's stream may still read it and cannot be tracked
+                FQ_HIP(hipDeviceSynchronize());
+            } else {
+                for (hipStream_t u : pe.users) {
+                    if (u == stream || !is_own_stream(u))
+                        continue;
+                    FQ_HIP(hipEventRecord(pe.ready, u));
+                    FQ_HIP(hipStreamWaitEvent(stream, pe.ready, 0));
+                }
+            }
+            pe.users.clear();
+            pe.foreign = false;
+        }
+        G->plan_epoch++;
+    }
+    if (!reused) {
+        size_t cap = 4096;
+        while (cap < total)
+            cap *= 2;
+        pe.cap = cap;
+        if (hipMalloc(&pe.dmem, cap) != hipSuccess ||
+            hipHostMalloc(&pe.staging, cap, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&pe.ready, hipEventDisableTiming) != hipSuccess) {
+            hipGetLastError();
+            plan_free(pe);
+            return set_err(FQ_ERR_OOM, "plan allocation of %zu bytes failed", cap);
+        }
     }
     memcpy(pe.staging, blob.data(), total);
     pe.stream = stream;
@@ -541,7 +672,11 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream
     dev.hh_u = (const int *)(b + o_hu);
     dev.hh_v = (const int *)(b + o_hv);
     pe.dev = dev;
-    G->plans[key] = pe;
+    pe.users.clear();
+    pe.foreign = false;
+    PlanEntry &ne = G->plans[key];
+    ne = pe;
+    note_user(ne);
     *out = dev;
     return FQ_OK;
 }
@@ -616,6 +751,7 @@ extern "C" int fq_state_create(int N, int D, int B, int C, int dtype, fq_state *
         fq_state_destroy(st);
         return set_err(FQ_ERR_OOM, "scratch allocation failed");
     }
+    own_stream(st->own, true);
     st->stream = st->own;
     *out = st;
     return FQ_OK;
@@ -645,8 +781,10 @@ extern "C" int fq_state_destroy(fq_state *st)
         budget_give(st->bytes);
     hipFree(st->aux);
     hipFree(st->d_grad);
-    if (st->own)
+    if (st->own) {
+        own_stream(st->own, false);
         hipStreamDestroy(st->own);
+    }
     delete st;
     return FQ_OK;
 }
@@ -912,15 +1050,9 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     if (int rc = pass_args(st->N, st->dtype, tile, gates, cost, z2, a, geom))
         return rc;
     const int a_log = geom.a_log;
-    double s, c;
-    sincos(beta / 2, &s, &c);
-    a.rx_c = c;
-    a.rx_s = s;
-    if (pre_beta) {
-        sincos(*pre_beta / 2, &s, &c);
-        a.pre_c = c;
-        a.pre_s = s;
-    }
+    rx_coeffs(beta, st->dtype, &a.rx_c, &a.rx_s);
+    if (pre_beta)
+        rx_coeffs(*pre_beta, st->dtype, &a.pre_c, &a.pre_s);
     if (cost) {
         if (int rc = get_plan(g, geom, &a.plan, st->stream))
             return rc;
@@ -944,13 +1076,12 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
 static int carriers_for(int dtype) { return dtype == FQ_C128 ? 3 : 4; }
 
 // Tile width of the fused and strided passes: the LDS limit by default;
-// FQ_TILE_Q (environment) may lower it (smaller tiles: two work-groups per CU).
+// FQ_TILE_Q (environment, read per call) may lower it (smaller tiles: two
+// work-groups per CU).
 static int pass_tile_q(int dtype)
 {
-    static const int env_q = [] {
-        const char *e = getenv("FQ_TILE_Q");
-        return e ? atoi(e) : 0;
-    }();
+    const char *e = getenv("FQ_TILE_Q");
+    const int env_q = e ? atoi(e) : 0;
     const int mx = tile_max_q(dtype);
     return env_q > 0 && env_q < mx ? env_q : mx;
 }
@@ -961,27 +1092,21 @@ struct FusedPass {
 };
 
 // Qubits per block when the strided passes' new qubits are dealt out (see
-// cover_passes): 2 by default; FQ_PASS_BLOCK (environment) selects another,
-// 0 for contiguous blocks.
+// cover_passes): 2 by default; FQ_PASS_BLOCK (environment, read per call)
+// selects another, 0 for contiguous blocks.
 static int pass_block()
 {
-    static const int blk = [] {
-        const char *e = getenv("FQ_PASS_BLOCK");
-        return e ? std::max(0, atoi(e)) : 2;
-    }();
-    return blk;
+    const char *e = getenv("FQ_PASS_BLOCK");
+    return e ? std::max(0, atoi(e)) : 2;
 }
 
 // Whether the carried low qubits' RX moves out of the cost pass (see
-// cover_passes): on by default; FQ_CARRIER_GATES=0 (environment) keeps every
-// low qubit's RX in pass 0.
+// cover_passes): on by default; FQ_CARRIER_GATES=0 (environment, read per
+// call) keeps every low qubit's RX in pass 0.
 static bool carrier_gates()
 {
-    static const bool on = [] {
-        const char *e = getenv("FQ_CARRIER_GATES");
-        return e ? atoi(e) != 0 : true;
-    }();
-    return on;
+    const char *e = getenv("FQ_CARRIER_GATES");
+    return e ? atoi(e) != 0 : true;
 }
 
 // The tile passes of one fused layer, covering every qubit once.  Pass 0 is
@@ -1321,6 +1446,13 @@ extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, co
         if (!std::isfinite(gammas[p]) || !std::isfinite(betas[p]))
             return set_err(FQ_ERR_ARG, "angles must be finite");
     const int N = st->N;
+    // launch-control amplitude of this evaluation (restored on return)
+    struct AmpScale {
+        fq_state *st;
+        double saved;
+        ~AmpScale() { st->init_amp = saved; }
+    } amp_guard{st, st->init_amp};
+    st->init_amp *= rx_norm_scale(N, P, betas, st->dtype);
     if (schedule == FQ_SCHED_REFERENCE) {
         // src/Weighted.cpp:97-137
         std::vector<int> sizes;
@@ -1502,10 +1634,7 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
             if (int rc = pass_args(N, st->dtype, ps[i].tile, ps[i].gates, costbw,
                                    z2 ? (mirror ? Z2_MIRROR : Z2_LOWER) : Z2_OFF, a, geom, a_log))
                 return rc;
-            double sn, cs;
-            sincos(-betas[k] / 2, &sn, &cs);
-            a.rx_c = cs;
-            a.rx_s = sn;
+            rx_coeffs(-betas[k], st->dtype, &a.rx_c, &a.rx_s);
             double *ctab = nullptr;
             if (costbw) {
                 if (int rc = get_plan(g, geom, &a.plan, st->stream))
@@ -1833,6 +1962,7 @@ struct SlotHost {
     int P = 0, layer = 0;
     int phase = 0;  // 0 free, 1 active, 2 finished (result pending collection)
     std::vector<double> gammas, betas;
+    double amp_scale = 1.0;   // rx_norm_scale of the request
     PlanDev plan{}, eplan{};  // cost and <C> plans of g, fetched at assignment
     uint64_t gversion = 0;    // g->version the plans belong to
     uint64_t gepoch = 0;      // g->plan_epoch the plan pointers belong to
@@ -1890,8 +2020,10 @@ static void slab_free(fq_slab *sb)
     hipHostFree(sb->h_res);
     if (sb->done)
         hipEventDestroy(sb->done);
-    if (sb->stream)
+    if (sb->stream) {
+        own_stream(sb->stream, false);
         hipStreamDestroy(sb->stream);
+    }
     delete sb;
 }
 
@@ -2067,6 +2199,7 @@ extern "C" int fq_slab_create_sched(int N, int nslots, int dtype, int schedule, 
         slab_free(sb);
         return set_err(FQ_ERR_OOM, "slab scratch allocation failed");
     }
+    own_stream(sb->stream, true);
     memset(sb->h_params, 0, sizeof(SlotParams) * nslots);
     *out = sb;
     return FQ_OK;
@@ -2126,6 +2259,7 @@ extern "C" int fq_slab_assign(fq_slab *sb, int slot, fq_graph *g, int P, const d
     h.phase = 1;
     h.gammas.assign(gammas, gammas + P);
     h.betas.assign(betas, betas + P);
+    h.amp_scale = rx_norm_scale(sb->N, P, betas, sb->dtype);
     return FQ_OK;
 }
 
@@ -2182,12 +2316,9 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         if (h.phase != 1)
             continue;
         const int L = h.layer;
-        double sn, cs;
-        sincos(h.betas[L] / 2, &sn, &cs);
-        sp.rx_c = cs;
-        sp.rx_s = sn;
+        rx_coeffs(h.betas[L], sb->dtype, &sp.rx_c, &sp.rx_s);
         sp.gamma = h.gammas[L];
-        sp.init_amp = sb->init_amp;
+        sp.init_amp = sb->init_amp * h.amp_scale;
         sp.active = 1;
         sp.cost_first = L == 0;
         sp.want_expect = L + 1 == h.P;

@@ -165,4 +165,13 @@ hipError_t launch_slab_expect(int dtype, int a_log, const void *sv, const TileAr
 int tile_max_q(int dtype);
 size_t tile_lds_bytes(int dtype, int Q);
 
+// RX(beta) coefficients c = cos(beta/2), s = sin(beta/2) (src/rx.cu:121) as
+// the kernels of amplitude type dtype apply them.  FQ_C128: the FP64 values.
+// FQ_C64: an FP32 pair (returned as doubles, exact when cast back) chosen so
+// that c^2 + s^2 = 1 as nearly as FP32 allows.  Rounding cos and sin to FP32
+// independently leaves c^2 + s^2 - 1 up to ~1e-7, and RX scales the norm of
+// the state by exactly c^2 + s^2, so that error compounds over every qubit
+// and layer into a drift of N*P*(c^2 + s^2 - 1) in <C>.
+void rx_coeffs(double beta, int dtype, double *c, double *s);
+
 }  // namespace fq
