@@ -20,9 +20,9 @@ schedule (layers on the z_{N-1} = 0 half, psi(b) = psi(~b); the full state is
 written after the last layer) for N=30 and N=34.
 
 cpu_baseline: the oracle (the C restatement of the reference's layer loop,
-complex128, OpenMP) times one layer of the same N=30 instance on a bounded
-sample: the first 1/8 of each operation's independent work units (oracle
-oq_layer_sample); layers/s = 1 / (8 * sample seconds).
+complex128, OpenMP, OMP_PROC_BIND=close) on the host cores, run first, in a
+child process: the first two full layers of the same N=30 instance timed
+around the layer loop alone (value: layers/s), and the whole N=20 P=5 loop.
 """
 from __future__ import annotations
 
@@ -90,40 +90,34 @@ def measure(name, dtype, schedule, steps, warmup, C=None, dominant=None):
     return res
 
 
-def cpu_baseline(f=3):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
+def cpu_baseline(n30_layers=2):
+    """The oracle (the C restatement of the reference's layer loop, complex128,
+    OpenMP) on the host cores, in a child process started before this process
+    touches the GPU: OMP_PROC_BIND=close, OMP_NUM_THREADS as set (else the
+    physical cores this process may use).  Times full layers with a monotonic
+    clock around the layer loop alone (src/Weighted.cpp:113,140)."""
+    import subprocess
 
-    import pyoracle as O
-    from forqaoa import graphs as G
-
-    w = G.workload("n30_gnp05")
-    N, C = w["N"], w["C"]
-    sizes = [min(C, N - i) for i in range(0, N, C)]
-    fmax = min([N - min(C, N)] + [N - 2 * k for k in sizes[1:]])
-    f = min(f, fmax)
-    W = O.dense_w(N, w["edges"])
-    sv = np.zeros(1 << N, dtype=np.complex128)  # pages materialise on first touch
-    O.layer_sample(N, C, W, w["gammas"][1], w["betas"][1], sv, f)  # first touch, untimed
-    t0 = time.perf_counter()
-    O.layer_sample(N, C, W, w["gammas"][1], w["betas"][1], sv, f)
-    t1 = time.perf_counter()
-    del sv
-    full = (t1 - t0) * (1 << f)
-    model = "unknown"
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return dict(value=1.0 / full, unit="layers/s", cores=O.num_threads(), kind="port", cpu_model=model,
-                host_cpus=os.cpu_count(),
-                sample=(f"oracle (C restatement of src/Weighted.cpp:115-137, complex128, OpenMP) on one "
-                        f"layer of N=30 G(30,0.5) P=5 C=13, first 1/{1 << f} of each operation's work units "
-                        f"({t1 - t0:.2f} s), scaled x{1 << f}: {full:.1f} s per full layer"))
+    env = dict(os.environ)
+    env["OMP_PROC_BIND"] = "close"
+    if not env.get("OMP_NUM_THREADS"):
+        env["OMP_NUM_THREADS"] = str(len(os.sched_getaffinity(0)))
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"),
+                          "--n30-layers", str(n30_layers)], env=env, capture_output=True, text=True,
+                         timeout=900)
+    if out.returncode != 0:
+        raise RuntimeError(f"cpu_baseline.py failed ({out.returncode}): {out.stderr[-500:]}")
+    d = json.loads(out.stdout.strip().splitlines()[-1])
+    n30, n20 = d["n30"], d["n20"]
+    return dict(value=n30["layers_per_s"], unit="layers/s", cores=d["threads"], kind="port",
+                cpu_model=d["cpu_model"], physical_cores_allowed=d["physical_cores_allowed"],
+                logical_cpus_allowed=d["logical_cpus_allowed"], omp_proc_bind=d["omp_proc_bind"],
+                n20_p5_layers_per_s=n20["layers_per_s"], n20_p5_seconds=n20["seconds"],
+                sample=(f"oracle (C restatement of src/Weighted.cpp:97-137, complex128, OpenMP, "
+                        f"{d['threads']} threads, OMP_PROC_BIND=close) timed around the layer loop: the first "
+                        f"{n30['P']} full layers of N=30 G(30,0.5) C=13 (launch-control layer, then "
+                        f"read-modify-write layers) in {n30['seconds']:.1f} s; no scaling. Also the whole "
+                        f"N=20 P=5 C=12 loop: {n20['seconds']:.3f} s"))
 
 
 def serving_bench(n_sat=10000, n_base=150, n_lat=600, seed=20251207):
@@ -194,6 +188,25 @@ def serving_bench(n_sat=10000, n_base=150, n_lat=600, seed=20251207):
         latency_rate_per_s=rate, latency_sample=n_lat, **lat)
 
 
+def serving_saturated(n, seed=20251207):
+    """The first n requests of the serving stream, all queued at t=0, through
+    the server (complex64, 200 GB budget, half-state slabs)."""
+    from forqaoa.serve import Server, bytes_floor, make_stream
+
+    full = make_stream(10000, seed=seed)
+    sub = type(full)(graphs=full.graphs, requests=full.requests[:n])
+    for r in sub.requests:
+        r.arrival = 0.0
+    srv = Server(dtype="c64", budget_bytes=200e9)
+    t0 = time.perf_counter()
+    done = srv.run(sub, realtime=False)
+    secs = time.perf_counter() - t0
+    srv.close()
+    floor = bytes_floor(full, sub.requests, "c64", "z2")
+    return dict(requests=n, seconds=secs, evals_per_s=len(done) / secs,
+                bandwidth_floor_evals_per_s=n / (floor / (HBM_PEAK_GBS * 1e9)))
+
+
 def adjoint_bench(n_sets=16, seed=2026):
     """The adjoint workload: N=26, P=8, seeded 4-regular graph, complex64;
     for each of 16 random angle sets one forward evaluation (half-state
@@ -251,7 +264,17 @@ def main():
     ap.add_argument("--workload", default="n30_gnp05")
     ap.add_argument("--dtype", default="c128")
     ap.add_argument("--schedule", default="fused")
+    ap.add_argument("--only", choices=["adjoint", "serving"],
+                    help="run one part alone and print its record (for profiling): the adjoint "
+                         "workload, or the first --requests requests of the serving stream queued at t=0")
+    ap.add_argument("--requests", type=int, default=300)
     args = ap.parse_args()
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps({"serving_saturated": serving_saturated(args.requests)}), flush=True)
+        return
     if args.calibrate_copy:
         from forqaoa import State
 
@@ -268,6 +291,14 @@ def main():
               "transposition is not part of it (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    cpu = None
+    if not args.no_cpu:
+        # before anything here touches the GPU: the oracle runs in a child process
+        try:
+            cpu = cpu_baseline()
+            log(f"cpu baseline: {cpu['value']:.4g} layers/s on {cpu['cores']} threads")
+        except Exception as e:
+            cpu = dict(error=str(e))
     main_res = measure(args.workload, args.dtype, args.schedule, args.steps, args.warmup)
     log(f"main: {main_res['layers_per_s']:.2f} layers/s, layer HBM frac {main_res['layer_hbm_frac']:.3f}")
     # what simple streaming kernels reach over a buffer of the state's size
@@ -311,13 +342,6 @@ def main():
                 f"one-at-a-time, p50 {serving['p50_ms']:.1f} ms p99 {serving['p99_ms']:.1f} ms")
         except Exception as e:
             serving = dict(error=repr(e))
-    cpu = None
-    if not args.no_cpu:
-        try:
-            cpu = cpu_baseline()
-            log(f"cpu baseline: {cpu['value']:.4g} layers/s on {cpu['cores']} threads")
-        except Exception as e:
-            cpu = dict(error=str(e))
 
     dom_name = "fused" if args.schedule == "fused" else "cost"
     dom = main_res["kernels"].get(dom_name, {})

@@ -16,6 +16,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -187,6 +188,21 @@ int oq_relabel(int N, double *W, int out, int in, int k)
 int oq_run_weighted(int N, int C, int P, double *W, const double *gammas,
                     const double *betas, oq_cplx *sv, int *qmap)
 {
+    return oq_run_weighted_timed(N, C, P, W, gammas, betas, sv, qmap, NULL);
+}
+
+/* MEASURET_START / MEASURET_END (include/utils.h:29-37, src/Weighted.cpp:113,
+ * 140) bracket the P-layer loop only; a monotonic clock here. */
+static double oq_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int oq_run_weighted_timed(int N, int C, int P, double *W, const double *gammas,
+                          const double *betas, oq_cplx *sv, int *qmap, double *seconds)
+{
     if (N < 1 || C < 1 || P < 0)
         return 1;
     const double init_amp = oq_init_amp(N);
@@ -199,6 +215,7 @@ int oq_run_weighted(int N, int C, int P, double *W, const double *gammas,
         targets[i] = i;
     for (int p = 0; p < N; p++)
         qmap[p] = p;
+    const double t0 = oq_now();
     for (int p = 0; p < P; p++) {
         oq_cost_layer(N, sv, W, gammas[p], p == 0, init_amp);
         int swap_in = 0;
@@ -220,6 +237,8 @@ int oq_run_weighted(int N, int C, int P, double *W, const double *gammas,
             }
         }
     }
+    if (seconds)
+        *seconds = oq_now() - t0;
     return 0;
 }
 
@@ -352,109 +371,4 @@ void oq_to_logical(int N, const oq_cplx *sv, const int *qmap, oq_cplx *out)
             L |= (((uint64_t)b >> p) & 1u) << qmap[p];
         out[L] = sv[b];
     }
-}
-
-/* Sampled variants of the three operations: same loops as above over the
- * first `units` work units. */
-static void cost_units(int N, oq_cplx *sv, const double *flat, double gamma, int64_t units)
-{
-#pragma omp parallel for schedule(static)
-    for (int64_t t = 0; t < units; t++) {
-        const uint64_t b = (uint64_t)t;
-        double sum = 0;
-        int cnt = 0;
-        for (int i = 0; i < N; i++) {
-            for (int j = i + 1; j < N; j++) {
-                if (flat[cnt] != 0) {
-                    if (((b >> i) & 1u) ^ ((b >> j) & 1u))
-                        sum += flat[cnt];
-                    else
-                        sum -= flat[cnt];
-                }
-                cnt++;
-            }
-        }
-        double s, c;
-        sincos(sum * gamma / 2, &s, &c);
-        const oq_cplx in = sv[t];
-        sv[t].re = in.re * c - in.im * s;
-        sv[t].im = in.re * s + in.im * c;
-    }
-}
-
-static void rx_units(int chunk, oq_cplx *sv, int g, double beta, int64_t units)
-{
-    double s, c;
-    sincos(beta / 2, &s, &c);
-    const int64_t half = (int64_t)1 << (chunk - 1);
-#pragma omp parallel for schedule(static)
-    for (int64_t ch = 0; ch < units; ch++) {
-        oq_cplx *base = sv + (ch << chunk);
-        for (int target = 0; target < g; target++) {
-            for (int64_t u = 0; u < half; u++) {
-                const uint64_t up = insert_zero((uint64_t)u, target);
-                const uint64_t lo = up + (1ull << target);
-                const oq_cplx q0 = base[up];
-                const oq_cplx q1 = base[lo];
-                oq_cplx o0, o1;
-                o0.re = q0.re * c + q1.im * s;
-                o0.im = q0.im * c - q1.re * s;
-                o1.re = q0.im * s + q1.re * c;
-                o1.im = -q0.re * s + q1.im * c;
-                base[up] = o0;
-                base[lo] = o1;
-            }
-        }
-    }
-}
-
-static void swap_units(oq_cplx *sv, int in, int k, int64_t units)
-{
-    const int64_t ns = (int64_t)1 << k;
-#pragma omp parallel for collapse(2) schedule(dynamic, 64)
-    for (int64_t r = 0; r < units; r++) {
-        for (int64_t s1 = 0; s1 < ns; s1++) {
-            uint64_t gidx = insert_zeros((uint64_t)r, 0, k);
-            gidx = insert_zeros(gidx, in, k);
-            for (int64_t s2 = s1 + 1; s2 < ns; s2++) {
-                const uint64_t up = ((uint64_t)s1 << in) + (uint64_t)s2 + gidx;
-                const uint64_t lo = (uint64_t)s1 + ((uint64_t)s2 << in) + gidx;
-                const oq_cplx tmp = sv[up];
-                sv[up] = sv[lo];
-                sv[lo] = tmp;
-            }
-        }
-    }
-}
-
-static int64_t frac_units(int logn, int f)
-{
-    return logn - f >= 0 ? (int64_t)1 << (logn - f) : 1;
-}
-
-int oq_layer_sample(int N, int C, double *W, double gamma, double beta, oq_cplx *sv, int f)
-{
-    if (N < 2 || C < 1 || f < 0)
-        return 1;
-    const int chunk = C < N ? C : N;
-    const int npairs = N * (N - 1) / 2;
-    double *flat = (double *)malloc(sizeof(double) * (npairs > 0 ? npairs : 1));
-    oq_flatten_upper(N, W, flat);
-    cost_units(N, sv, flat, gamma, frac_units(N, f));
-    free(flat);
-    int sizes[64];
-    int R = 0;
-    for (int i = 0; i < N; i += C)
-        sizes[R++] = (i + C <= N) ? C : N - i;
-    int swap_in = 0;
-    for (int i = 0; i < R; i++) {
-        rx_units(chunk, sv, sizes[i], beta, frac_units(N - chunk, f));
-        if (i < R - 1) {
-            swap_in += sizes[i];
-            const int k = sizes[i + 1];
-            swap_units(sv, swap_in, k, frac_units(N - 2 * k, f));
-            oq_relabel(N, W, 0, swap_in, k);
-        }
-    }
-    return 0;
 }

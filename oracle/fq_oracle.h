@@ -60,6 +60,12 @@ int oq_relabel(int N, double *W, int out, int in, int k);
 int oq_run_weighted(int N, int C, int P, double *W, const double *gammas,
                     const double *betas, oq_cplx *sv, int *qmap);
 
+/* oq_run_weighted, and *seconds receives the monotonic-clock time of the
+ * P-layer loop alone, where MEASURET_START / MEASURET_END sit in the driver
+ * (src/Weighted.cpp:113,140). */
+int oq_run_weighted_timed(int N, int C, int P, double *W, const double *gammas,
+                          const double *betas, oq_cplx *sv, int *qmap, double *seconds);
+
 /* Cut(z) = sum_{i<j} W[i][j] * [z_i != z_j] for one basis index. */
 double oq_cut_value(int N, const double *W, uint64_t b);
 
@@ -75,15 +81,6 @@ void oq_topk(int N, const oq_cplx *sv, int k, uint64_t *idx, double *prob);
 /* Physical -> logical reordering: out[L(b)] = sv[b] with bit p of b moved to
  * bit qmap[p]. */
 void oq_to_logical(int N, const oq_cplx *sv, const int *qmap, oq_cplx *out);
-
-/* Timing sample of one layer p >= 1 of the driver loop at full size N (cost
- * read-modify-write, then RX / swap / relabel per subcircuit as
- * src/Weighted.cpp:119-137), restricted to the first 1/2^f of each
- * operation's independent work units: amplitudes of the cost phase, 2^C
- * chunks of each RX pass, rest indices of each swap.  Every unit costs the
- * same, so a full layer takes 2^f times as long.  sv must span 2^N
- * amplitudes (only the touched pages are materialised).  W is relabelled. */
-int oq_layer_sample(int N, int C, double *W, double gamma, double beta, oq_cplx *sv, int f);
 
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int oq_num_threads(void);

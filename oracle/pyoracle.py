@@ -38,6 +38,8 @@ def lib():
         L.oq_relabel.restype = ctypes.c_int
         L.oq_run_weighted.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, vp, ip]
         L.oq_run_weighted.restype = ctypes.c_int
+        L.oq_run_weighted_timed.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, vp, ip, dp]
+        L.oq_run_weighted_timed.restype = ctypes.c_int
         L.oq_cut_value.argtypes = [ctypes.c_int, dp, ctypes.c_uint64]
         L.oq_cut_value.restype = ctypes.c_double
         L.oq_expect_cut.argtypes = [ctypes.c_int, vp, dp]
@@ -47,9 +49,6 @@ def lib():
         L.oq_topk.argtypes = [ctypes.c_int, vp, ctypes.c_int, up, dp]
         L.oq_to_logical.argtypes = [ctypes.c_int, vp, ip, vp]
         L.oq_num_threads.restype = ctypes.c_int
-        L.oq_layer_sample.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, vp,
-                                      ctypes.c_int]
-        L.oq_layer_sample.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -124,6 +123,23 @@ def run_weighted(N, C, edges, gammas, betas):
     return sv, qmap, W
 
 
+def run_weighted_timed(N, C, edges, gammas, betas):
+    """run_weighted, also returning the seconds of the P-layer loop alone
+    (where MEASURET_START / MEASURET_END sit, src/Weighted.cpp:113,140)."""
+    P = len(gammas)
+    assert len(betas) == P
+    W = dense_w(N, edges)
+    g = np.ascontiguousarray(gammas, dtype=np.float64)
+    b = np.ascontiguousarray(betas, dtype=np.float64)
+    sv = np.zeros(1 << N, dtype=np.complex128)
+    qmap = np.zeros(N, dtype=np.int32)
+    secs = np.zeros(1, dtype=np.float64)
+    rc = lib().oq_run_weighted_timed(N, C, P, _dp(W), _dp(g), _dp(b), sv.ctypes.data, _ip(qmap), _dp(secs))
+    if rc:
+        raise ValueError(f"oq_run_weighted_timed failed ({rc})")
+    return sv, qmap, W, float(secs[0])
+
+
 def to_logical(sv, qmap):
     N = len(qmap)
     q = np.ascontiguousarray(qmap, dtype=np.int32)
@@ -158,12 +174,3 @@ def topk(sv, k):
 
 def num_threads():
     return lib().oq_num_threads()
-
-
-def layer_sample(N, C, W, gamma, beta, sv, f):
-    """One reference-schedule layer at size N on the first 1/2^f of each
-    operation's work units (oq_layer_sample)."""
-    assert W.flags.c_contiguous and W.dtype == np.float64
-    rc = lib().oq_layer_sample(N, C, _dp(W), gamma, beta, sv.ctypes.data, f)
-    if rc:
-        raise ValueError("oq_layer_sample failed")
