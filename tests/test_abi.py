@@ -231,3 +231,28 @@ def test_boundary_pass_gets_no_carried_rx(dtype, schedule):
             if q < q0:
                 assert all(q >= carriers(ps[k][0]) for k in plain), (N, q, ps)
         assert any(q < q0 for k in plain for q in ps[k][1]), (N, ps)
+
+
+@pytest.mark.parametrize("env", [{"FQ_TILE_Q": "10"}, {"FQ_TILE_Q": "11"}, {"FQ_TILE_Q": "12"},
+                                 {"FQ_PASS_BLOCK": "0"}, {"FQ_PASS_BLOCK": "1"}, {"FQ_PASS_BLOCK": "3"},
+                                 {"FQ_PASS_BLOCK": "4"}, {"FQ_CARRIER_GATES": "0"}])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("schedule", ["fused", "z2"])
+def test_layer_passes_under_tuning_knobs(env, dtype, schedule, monkeypatch):
+    # host-only: the knobs (read on every call) still give one RX per qubit per
+    # layer, tiles of distinct positions, and the mirror qubit last
+    from forqaoa import layer_passes
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    qmax = min(13 if dtype == "c128" else 14, int(env.get("FQ_TILE_Q", "99")))
+    for N in range(3, 37):
+        ps = layer_passes(N, dtype, schedule)
+        Q = min(qmax, N - 1 if schedule == "z2" else N)
+        assert ps[0][0] == list(range(Q)), (N, ps[0])
+        assert sorted(q for _, g in ps for q in g) == list(range(N)), (N, ps)
+        for tile, gates in ps:
+            assert len(tile) == Q and len(set(tile)) == Q and set(gates) <= set(tile)
+        if env.get("FQ_CARRIER_GATES") == "0":
+            assert set(range(Q)) <= set(ps[0][1]), (N, ps)
+        if schedule == "z2":
+            assert ps[-1][0][-1] == N - 1
