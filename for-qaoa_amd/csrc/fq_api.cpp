@@ -1375,61 +1375,68 @@ static LayerPlan layer_plan(int N, int dtype, bool z2)
     return lp;
 }
 
-// The tile passes of P layers (pass 0 the low contiguous tile; z2: half-state
-// passes, the last one the mirror pass).  In layer order, layer p runs passes
-// 0..m-1 with its cost phase in pass 0.  Pipelined: the cost phase is
-// diagonal, so it can run in any pass; the pass that ends layer p applies
-// RX(beta_p), the phase of gamma_{p+1} and RX(beta_{p+1}) on its qubits,
-// which opens layer p+1.  Layers end alternately on end1 and pass 0, so every
-// layer after the first runs m-1 passes: 1 + P(m-1) passes in all instead of
-// Pm.  Every RX of a layer still follows its cost phase and precedes the next
-// one, and RX gates on distinct qubits commute.
+// The tile passes of layer p of P (pass 0 the low contiguous tile; z2:
+// half-state passes, the last one the mirror pass).  In layer order, layer p
+// runs passes 0..m-1 with its cost phase in pass 0.  Pipelined: the cost
+// phase is diagonal, so it can run in any pass; the pass that ends layer p
+// applies RX(beta_p), the phase of gamma_{p+1} and RX(beta_{p+1}) on its
+// qubits, which opens layer p+1.  Layers end alternately on end1 and pass 0,
+// so every layer after the first runs m-1 passes: 1 + P(m-1) passes in all
+// instead of Pm.  Every RX of a layer still follows its cost phase and
+// precedes the next one, and RX gates on distinct qubits commute.  Layer p
+// expects layers 0..p-1 to have run with the same plan.  expand: the last
+// layer's mirror pass also writes the upper half of the full state.
+static int run_layer(fq_state *st, fq_graph *g, const LayerPlan &lp, int p, int P, const double *gammas,
+                     const double *betas, bool z2, bool expand)
+{
+    const std::vector<FusedPass> &ps = lp.ps;
+    const int m = (int)ps.size();
+    const bool last = p + 1 == P;
+    auto mode = [&](int i) {
+        if (!z2)
+            return (int)Z2_OFF;
+        return i + 1 < m ? (int)Z2_LOWER : last && expand ? (int)Z2_EXPAND : (int)Z2_MIRROR;
+    };
+    auto plain = [&](int i) {
+        return tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, 0.0, false, false, FQ_K_FUSED, mode(i));
+    };
+    if (lp.end1 < 0) {
+        for (int i = 0; i < m; ++i) {
+            const int rc = i == 0 ? tile_pass(st, ps[0].tile, ps[0].gates, betas[p], g, gammas[p], true, p == 0,
+                                              FQ_K_FUSED, mode(0))
+                                  : plain(i);
+            if (rc)
+                return rc;
+        }
+        return FQ_OK;
+    }
+    const int e = p % 2 == 0 ? lp.end1 : 0;   // the pass that ends layer p (the other one opened it)
+    if (p == 0)
+        if (int rc = tile_pass(st, ps[0].tile, ps[0].gates, betas[0], g, gammas[0], true, true, FQ_K_FUSED,
+                               mode(0)))
+            return rc;
+    if (last)
+        if (int rc = plain(e))
+            return rc;
+    for (int i = 1; i < m; ++i) {
+        if (i == lp.end1)
+            continue;
+        if (int rc = plain(i))
+            return rc;
+    }
+    if (!last)
+        if (int rc = tile_pass(st, ps[e].tile, ps[e].gates, betas[p + 1], g, gammas[p + 1], true, false,
+                               FQ_K_FUSED, mode(e), &betas[p]))
+            return rc;
+    return FQ_OK;
+}
+
 static int run_layers(fq_state *st, fq_graph *g, int P, const double *gammas, const double *betas, bool z2)
 {
     const LayerPlan lp = layer_plan(st->N, st->dtype, z2);
-    const std::vector<FusedPass> &ps = lp.ps;
-    const int m = (int)ps.size();
-    auto mode = [&](int i, bool last) {
-        if (!z2)
-            return (int)Z2_OFF;
-        return i + 1 < m ? (int)Z2_LOWER : last ? (int)Z2_EXPAND : (int)Z2_MIRROR;
-    };
-    auto plain = [&](int i, int p) {
-        return tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, 0.0, false, false, FQ_K_FUSED,
-                         mode(i, p + 1 == P));
-    };
-    if (lp.end1 < 0) {
-        for (int p = 0; p < P; ++p)
-            for (int i = 0; i < m; ++i) {
-                const int rc = i == 0 ? tile_pass(st, ps[0].tile, ps[0].gates, betas[p], g, gammas[p], true,
-                                                  p == 0, FQ_K_FUSED, mode(0, p + 1 == P))
-                                      : plain(i, p);
-                if (rc)
-                    return rc;
-            }
-        return FQ_OK;
-    }
-    for (int p = 0; p < P; ++p) {
-        const bool last = p + 1 == P;
-        const int e = p % 2 == 0 ? lp.end1 : 0;   // the pass that ends layer p (the other one opened it)
-        if (p == 0)
-            if (int rc = tile_pass(st, ps[0].tile, ps[0].gates, betas[0], g, gammas[0], true, true, FQ_K_FUSED,
-                                   mode(0, last)))
-                return rc;
-        if (last)
-            if (int rc = plain(e, p))
-                return rc;
-        for (int i = 1; i < m; ++i) {
-            if (i == lp.end1)
-                continue;
-            if (int rc = plain(i, p))
-                return rc;
-        }
-        if (!last)
-            if (int rc = tile_pass(st, ps[e].tile, ps[e].gates, betas[p + 1], g, gammas[p + 1], true, false,
-                                   FQ_K_FUSED, mode(e, false), &betas[p]))
-                return rc;
-    }
+    for (int p = 0; p < P; ++p)
+        if (int rc = run_layer(st, g, lp, p, P, gammas, betas, z2, true))
+            return rc;
     return FQ_OK;
 }
 
@@ -1998,17 +2005,24 @@ struct fq_slab {
     std::vector<SlabPass> pass_args;
     TileArgs exp_args{};       // <C> over the virtual state
     int exp_alog = 1, exp_shift = 0;
-    // large slots: each slot's passes are separate tile-pass launches over
-    // its own 2^N (or 2^(N-1)) amplitudes instead of one packed launch
+    // large slots: each slot runs the single-state tile passes of fq_run
+    // (pipelined layers) over its own 2^N (or 2^(N-1)) amplitudes instead of
+    // one packed launch; views[k] is slot k seen as a state (its amplitudes,
+    // the slab's stream, its own phase-table buffer)
     bool per_slot = false;
-    std::vector<TileArgs> slot_pass;
-    std::vector<int> slot_pass_alog, slot_pass_z2;
+    LayerPlan slot_plan;
+    std::vector<fq_state *> views;
 };
 
 static void slab_free(fq_slab *sb)
 {
     if (sb->stream)
         hipStreamSynchronize(sb->stream);
+    for (fq_state *v : sb->views) {
+        hipFree(v->d_phase);
+        delete v;
+    }
+    sb->views.clear();
     if (sb->sv)
         budget_give(sb->bytes);
     hipFree(sb->sv);
@@ -2089,20 +2103,47 @@ static int slab_build_args(fq_slab *sb)
     // widths) fills the device on its own
     const int Q1 = (int)sb->passes[0].tile.size();
     sb->per_slot = Ns - Q1 >= 10;
-    sb->slot_pass.clear();
-    sb->slot_pass_alog.clear();
-    sb->slot_pass_z2.clear();
-    if (sb->per_slot) {
-        for (size_t i = 0; i < sb->passes.size(); ++i) {
-            const int z2 = !sb->z2 ? Z2_OFF : i + 1 == sb->passes.size() ? Z2_MIRROR : Z2_LOWER;
-            TileArgs a;
-            Geom geom;
-            if (int rc = pass_args(N, sb->dtype, sb->passes[i].tile, sb->passes[i].gates, i == 0, z2, a, geom))
-                return rc;
-            sb->slot_pass.push_back(a);
-            sb->slot_pass_alog.push_back(geom.a_log);
-            sb->slot_pass_z2.push_back(z2);
+    if (sb->per_slot)
+        sb->slot_plan = layer_plan(N, sb->dtype, sb->z2);
+    return FQ_OK;
+}
+
+// Per-slot state views of a per-slot slab (after its amplitudes and stream
+// exist), each with a phase-table buffer for the largest cost pass of the
+// layer plan, so no step allocates.
+static int slab_make_views(fq_slab *sb)
+{
+    const LayerPlan &lp = sb->slot_plan;
+    const int m = (int)lp.ps.size();
+    size_t need = 0;
+    for (int i = 0; i < m; ++i) {
+        if (i != 0 && i != lp.end1)
+            continue;
+        TileArgs a;
+        Geom geom;
+        const int z2 = !sb->z2 ? Z2_OFF : i + 1 < m ? Z2_LOWER : Z2_MIRROR;
+        if (int rc = pass_args(sb->N, sb->dtype, lp.ps[i].tile, lp.ps[i].gates, true, z2, a, geom))
+            return rc;
+        need = std::max(need, phase_table_entries(a, geom.a_log) * sizeof(double2));
+    }
+    for (int k = 0; k < sb->K; ++k) {
+        fq_state *v = new fq_state();
+        v->N = sb->N;
+        v->dtype = sb->dtype;
+        v->elem = sb->elem;
+        v->bytes = sb->slot_bytes;
+        v->sv = (char *)sb->sv + sb->slot_bytes * (size_t)k;
+        v->stream = sb->stream;
+        for (int q = 0; q < FQ_MAXN; ++q)
+            v->qmap[q] = q;
+        v->init_amp = sb->init_amp;
+        sb->views.push_back(v);
+        if (hipMalloc(&v->d_phase, need) != hipSuccess) {
+            hipGetLastError();
+            v->d_phase = nullptr;
+            return set_err(FQ_ERR_OOM, "slab phase-table allocation failed");
         }
+        v->phase_cap = need;
     }
     return FQ_OK;
 }
@@ -2200,6 +2241,11 @@ extern "C" int fq_slab_create_sched(int N, int nslots, int dtype, int schedule, 
         return set_err(FQ_ERR_OOM, "slab scratch allocation failed");
     }
     own_stream(sb->stream, true);
+    if (sb->per_slot)
+        if (int rc = slab_make_views(sb)) {
+            slab_free(sb);
+            return rc;
+        }
     memset(sb->h_params, 0, sizeof(SlotParams) * nslots);
     *out = sb;
     return FQ_OK;
@@ -2335,31 +2381,22 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         return FQ_OK;
     FQ_HIP(hipMemcpyAsync(sb->d_params, sb->h_params, sizeof(SlotParams) * K, hipMemcpyHostToDevice,
                           sb->stream));
-    FQ_HIP(launch_slab_phase_table(sb->cost_args, sb->cost_geom.a_log, sb->d_params, K, sb->d_tabs,
-                                   sb->tab_stride, sb->stream));
     if (sb->per_slot) {
+        // each slot one layer of fq_run's pipelined schedule on its own amplitudes
         for (int k = 0; k < K; ++k) {
             const SlotParams &sp = sb->h_params[k];
             if (!sp.active)
                 continue;
-            void *svk = (char *)sb->sv + sb->slot_bytes * (size_t)k;
-            const double2 *ptab = sb->d_tabs + sb->tab_stride * (size_t)k;
-            for (size_t i = 0; i < sb->slot_pass.size(); ++i) {
-                TileArgs a = sb->slot_pass[i];
-                a.rx_c = sp.rx_c;
-                a.rx_s = sp.rx_s;
-                a.gamma = sp.gamma;
-                a.init_amp = sp.init_amp;
-                a.has_rr = sp.has_rr;
-                const int al = sb->slot_pass_alog[i];
-                if (sb->slot_pass_z2[i] >= Z2_MIRROR)
-                    FQ_HIP(launch_tile_mirror(sb->dtype, al, false, svk, a, sb->stream));
-                else
-                    FQ_HIP(launch_tile(sb->dtype, al, i == 0, i == 0 && sp.cost_first, true, svk, a, ptab,
-                                       sb->stream));
-            }
+            SlotHost &h = sb->slots[k];
+            fq_state *v = sb->views[k];
+            v->init_amp = sp.init_amp;
+            if (int rc = run_layer(v, h.g, sb->slot_plan, h.layer, h.P, h.gammas.data(), h.betas.data(), sb->z2,
+                                   false))
+                return rc;
         }
     } else {
+        FQ_HIP(launch_slab_phase_table(sb->cost_args, sb->cost_geom.a_log, sb->d_params, K, sb->d_tabs,
+                                       sb->tab_stride, sb->stream));
         for (size_t i = 0; i < sb->pass_args.size(); ++i) {
             const SlabPass &sp = sb->pass_args[i];
             FQ_HIP(launch_slab_pass(sb->dtype, sp.a_log, i == 0, sp.mirror, sb->sv, sp.args, sb->d_params,

@@ -186,6 +186,7 @@ class Server:
         check(lib().fq_set_memory_budget(ctypes.c_size_t(int(budget_bytes))))
         self.slabs = {}
         self.graphs = {}
+        self.peak_in_use = in_use()  # HBM held by this process, sampled after every slab creation
 
     def graph(self, stream, key):
         g = self.graphs.get(key)
@@ -212,6 +213,7 @@ class Server:
                 self.slabs.pop(idle[0]).destroy()
                 continue
             self.slabs[N] = sb
+            self.peak_in_use = max(self.peak_in_use, in_use())
         return sb
 
     def run(self, stream, realtime=True, log=None):
@@ -299,21 +301,28 @@ def latency_stats(done):
 
 
 def bytes_floor(stream, requests, dtype="c64", schedule="fused"):
-    """Algorithmic HBM bytes of the slab schedule for these requests (layer 0
-    writes once, every other pass reads and writes; plus one <C> read).  The
-    half-state schedule stores and moves half of each state; its tile passes
-    cover N qubits with the first pass limited to N-1."""
+    """Algorithmic HBM bytes of the slab schedule for these requests: the
+    tile passes each request's slot runs (layer 0's first pass writes the
+    stored state once, every other pass reads and writes it) plus one <C>
+    read.  A half-state slab stores and moves half of each state.  Slots of
+    at least 2^10 first-pass tiles run fq_run's pipelined layers (1 + P(m-1)
+    passes for m passes per layer, when m >= 2 (full) or 3 (half state));
+    smaller slots run P*m packed passes."""
+    from . import layer_passes
+
     elem = 16 if DTYPES[dtype] == FQ_C128 else 8
     qmax = 13 if DTYPES[dtype] == FQ_C128 else 14
-    carriers = 3 if DTYPES[dtype] == FQ_C128 else 4
-    z2 = schedule == "z2"
+    m_of = {}
     tot = 0.0
     for r in requests:
-        half = z2 and r.N >= 3
-        S = (1 << (r.N - 1 if half else r.N)) * elem
-        q = min(qmax, r.N - 1 if half else r.N)
-        rest = r.N - q
-        per = q - min(carriers, q - 1)
-        npass = 1 + (0 if rest <= 0 else -(-rest // per))
-        tot += S * (2 * npass * r.P - 1) + S
+        half = schedule == "z2" and r.N >= 3
+        ns = r.N - 1 if half else r.N
+        S = (1 << ns) * elem
+        if r.N not in m_of:
+            m_of[r.N] = len(layer_passes(r.N, dtype, "z2" if half else "fused"))
+        m = m_of[r.N]
+        per_slot = ns - min(qmax, ns) >= 10
+        piped = per_slot and m >= (3 if half else 2)
+        npass = 1 + r.P * (m - 1) if piped else r.P * m
+        tot += S * (2 * npass - 1) + S
     return tot

@@ -34,14 +34,23 @@ from forqaoa import graphs as G  # noqa: E402
 
 
 def host_cpu():
-    """CPU model and physical cores of the CPUs this process may run on."""
+    """CPU model (lscpu's model name, family, model and stepping), physical
+    cores and logical CPUs this process may run on.  Called before the
+    OpenMP runtime starts: with OMP_PROC_BIND set it pins the calling thread
+    to one CPU."""
     model, cores = "unknown", None
+    allowed = os.sched_getaffinity(0)
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        info = {}
         for line in out.splitlines():
-            if line.startswith("Model name:"):
-                model = line.split(":", 1)[1].strip()
-        allowed = os.sched_getaffinity(0)
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info[k.strip()] = v.strip()
+        model = info.get("Model name", "unknown")
+        extra = ", ".join(f"{k.lower()} {info[k]}" for k in ("CPU family", "Model", "Stepping") if k in info)
+        if extra:
+            model = f"{model} ({extra})"
         table = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
                                check=True).stdout
         phys = set()
@@ -54,7 +63,7 @@ def host_cpu():
         cores = len(phys)
     except (OSError, subprocess.CalledProcessError, ValueError):
         pass
-    return model, cores
+    return model, cores, len(allowed)
 
 
 def timed(name, P):
@@ -68,11 +77,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n30-layers", type=int, default=2)
     args = ap.parse_args()
-    model, phys = host_cpu()
+    model, phys, logical = host_cpu()
     n20 = timed("n20_gnp05", 5)
     n30 = timed("n30_gnp05", args.n30_layers)
     print(json.dumps(dict(n30=n30, n20=n20, threads=O.num_threads(), cpu_model=model,
-                          physical_cores_allowed=phys, logical_cpus_allowed=len(os.sched_getaffinity(0)),
+                          physical_cores_allowed=phys, logical_cpus_allowed=logical,
                           omp_proc_bind=os.environ.get("OMP_PROC_BIND"))), flush=True)
 
 

@@ -1,15 +1,21 @@
 """Packed slabs and the continuous-batching server.
 
 Each request's <C> in the mix must equal the same request run alone through
-fq_run (fused schedule, its own state): within 1e-5 relative for complex64
+fq_run (fused schedule, its own state) and the oracle's <C> of the same
+request (the reference's layer loop, src/Weighted.cpp:97-137, where the
+request is small enough to run it here): within 1e-5 relative for complex64
 (the stated tolerance of the complex64 path) and 1e-12 for complex128; a
-slot's full state equals the solo state within the same tolerance.
+slot's full state equals the solo state within the same tolerance.  A slice
+of the serving workload itself (N 16..30, P 1..10, 200 GB budget) runs
+through the server with its N=29 and N=30 requests checked against solo runs
+and complex128.
 """
 import ctypes
 
 import numpy as np
 import pytest
 
+import pyoracle as O
 from forqaoa import FQError, Graph, State, _lib
 from forqaoa import graphs as G
 from forqaoa.serve import Server, Slab, in_use, latency_stats, make_stream, one_at_a_time
@@ -61,6 +67,17 @@ def test_slab_slots_match_solo_runs(dtype, schedule, N):
     sb.destroy()
 
 
+_ORACLE = {}
+
+
+def oracle_expect(N, edges, g, b):
+    key = (N, tuple(edges), tuple(g), tuple(b))
+    if key not in _ORACLE:
+        sv, _, W = O.run_weighted(N, min(N, 10), edges, g, b)
+        _ORACLE[key] = O.expect_cut(sv, W)
+    return _ORACLE[key]
+
+
 @pytest.mark.parametrize("schedule", ["z2", "fused"])
 @pytest.mark.parametrize("dtype", ["c64", "c128"])
 def test_server_results_equal_solo(dtype, schedule):
@@ -69,14 +86,18 @@ def test_server_results_equal_solo(dtype, schedule):
     srv = Server(dtype=dtype, budget_bytes=in_use() + 8e9, schedule=schedule)
     done = srv.run(stream, realtime=False)
     assert len(done) == 60
-    worst = 0.0
+    worst = worst_o = 0.0
     for r in done:
         N, edges = stream.graphs[r.graph_key]
         e_solo, _ = solo(N, edges, r.gammas, r.betas, dtype)
+        e_or = oracle_expect(N, edges, list(r.gammas), list(r.betas))
         worst = max(worst, abs(r.expect - e_solo) / abs(e_solo))
-        assert r.norm2 == pytest.approx(1.0, abs=1e-4 if dtype == "c64" else 1e-11)
-    print(f"server {dtype} {schedule}: max relative |<C>_mix - <C>_alone| = {worst:.2e}", flush=True)
+        worst_o = max(worst_o, abs(r.expect - e_or) / abs(e_or))
+        assert r.norm2 == pytest.approx(1.0, abs=1e-5 if dtype == "c64" else 1e-11)
+    print(f"server {dtype} {schedule}: max relative |<C>_mix - <C>_alone| = {worst:.2e}, "
+          f"|<C>_mix - <C>_oracle| = {worst_o:.2e}", flush=True)
     assert worst < TOL[dtype]
+    assert worst_o < TOL[dtype]
     # requests joined slabs that already had requests in flight
     assert any(r.joined_running > 0 for r in done)
     srv.close()
@@ -172,3 +193,61 @@ def test_slab_random_sizes_match_solo(case):
         sb.destroy()
         for gr in graphs:
             gr.destroy()
+
+
+def test_serving_workload_slice_under_budget():
+    # every 25th request of the serving workload's own stream (N 16..30,
+    # P 1..10, G(N, p) p in [0.2, 0.8], 80% resident-graph re-evaluations):
+    # 400 requests through half-state complex64 slabs under a 200 GB budget
+    full = make_stream(10000, seed=20251207)
+    reqs = full.requests[::25]
+    for r in reqs:
+        r.arrival = 0.0
+    stream = type(full)(graphs=full.graphs, requests=reqs)
+    base = in_use()
+    budget = base + 200e9
+    srv = Server(dtype="c64", budget_bytes=budget)
+    try:
+        done = srv.run(stream, realtime=False)
+        peak = srv.peak_in_use
+    finally:
+        srv.close()
+        _lib.check(_lib.lib().fq_set_memory_budget(0))
+    assert len(done) == len(reqs)
+    worst_norm = max(abs(r.norm2 - 1.0) for r in done)
+    print(f"slice: {len(done)} requests, N {min(r.N for r in done)}..{max(r.N for r in done)}, "
+          f"peak HBM {(peak - base) / 1e9:.1f} GB of 200, max |norm2 - 1| {worst_norm:.2e}", flush=True)
+    assert peak <= budget
+    assert worst_norm < 1e-5
+    # the mix against solo runs: every N=29 and N=30 request (up to 4 each),
+    # and one request of each other size; the N=30 ones also against complex128
+    by_n = {}
+    for r in done:
+        by_n.setdefault(r.N, []).append(r)
+    check = []
+    for N, rs in sorted(by_n.items()):
+        check += rs[:4] if N >= 29 else rs[:1]
+    assert any(r.N == 30 for r in check) and any(r.N == 29 for r in check)
+    worst = worst128 = 0.0
+    for r in check:
+        N, edges = full.graphs[r.graph_key]
+        e_solo = solo_expect(N, edges, r.gammas, r.betas, "c64")
+        worst = max(worst, abs(r.expect - e_solo) / abs(e_solo))
+        if N == 30:
+            e128 = solo_expect(N, edges, r.gammas, r.betas, "c128")
+            worst128 = max(worst128, abs(r.expect - e128) / abs(e128))
+    print(f"slice: {len(check)} requests vs solo runs: max rel diff {worst:.2e}; "
+          f"N=30 complex64 mix vs complex128 solo: {worst128:.2e}", flush=True)
+    assert worst < TOL["c64"]
+    assert worst128 < TOL["c64"]
+
+
+def solo_expect(N, edges, g, b, dtype):
+    st = State(N, C=min(N, 13 if dtype == "c128" else 14), dtype=dtype)
+    gr = Graph(N, edges)
+    try:
+        st.run(gr, g, b, "fused")
+        return st.expect_cut(gr)[0]
+    finally:
+        st.destroy()
+        gr.destroy()
