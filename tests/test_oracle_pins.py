@@ -277,3 +277,39 @@ def test_final_state_is_complement_symmetric(N, C, P, seed):
     if N <= 10:
         psi = R.dense_qaoa(N, edges, g, b)
         np.testing.assert_allclose(psi, psi[full ^ idx], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["n30_gnp05", "n30_reg3"])
+def test_large_fixtures_are_qaoa_states(name):
+    # the N=30 P=5 oracle fixtures (tests/golden/make_oracle_large.py): unit
+    # norm, <C> inside [0, W], and the complement symmetry psi(b) = psi(~b)
+    # of every QAOA state seen in the logical top-16: each bitstring's
+    # complement is there with the same probability (a wrong qubit map or
+    # logical index would break the pairing)
+    import os
+    d = GU.instances()[name]
+    with np.load(os.path.join(GU.GOLDEN, f"oracle_{name}.npz"), allow_pickle=False) as z:
+        fx = {k: z[k] for k in z.files}
+    N = d["N"]
+    assert abs(float(fx["norm2"]) - 1.0) < 1e-13
+    assert 0.0 < float(fx["expect_cut"]) < sum(w for _, _, w in d["edges"])
+    full = (1 << N) - 1
+    prob = {int(i): float(p) for i, p in zip(fx["top16_idx"], fx["top16_prob"])}
+    paired = [i for i in prob if (full ^ i) in prob]
+    assert len(paired) >= 14
+    for i in paired:
+        assert prob[i] == pytest.approx(prob[full ^ i], rel=1e-12)
+    assert len(fx["sample_idx"]) > 8000
+    amps = fx["sample_re"] + 1j * fx["sample_im"]
+    # sampled |psi|^2 average ~ 2^-N
+    assert np.mean(np.abs(amps) ** 2) * (1 << N) == pytest.approx(1.0, rel=0.2)
+
+
+def test_n26_fixture_expectations_in_range():
+    import os
+    d = GU.instances()["n26_reg4"]
+    with np.load(os.path.join(GU.GOLDEN, "oracle_n26_reg4.npz"), allow_pickle=False) as z:
+        ex = z["expect_cut"]
+        assert z["gammas"].shape == (16, d["P"])
+    W = sum(w for _, _, w in d["edges"])
+    assert np.all((ex > 0) & (ex < W))
