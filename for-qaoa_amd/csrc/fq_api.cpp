@@ -459,42 +459,18 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
 
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// Device cut plan of graph G for tile geometry g, cached per (geometry, W
-// content).  A new plan is uploaded asynchronously on `stream`; a cached plan
-// whose upload was queued on another stream makes `stream` wait for it.
-// With FQ_PLAN_CACHE plans cached, the least recently used one whose buffers
-// are large enough is replaced: `stream` first waits (on the device, through
-// an event) for the work queued so far on every stream that plan was handed
-// to, so no launch still reading it sees the new contents, and the host
-// blocks on nothing but that plan's own earlier upload.
-static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream)
+// The cut plan of graph G for tile geometry g as one blob (doubles first,
+// then ints, each array 16-byte aligned); dev receives the plan's scalars
+// (place_plan fills in the arrays' device addresses).
+struct PlanBlob {
+    std::vector<char> bytes;
+    PlanDev dev{};
+    size_t o_cH = 0, o_om = 0, o_cw = 0, o_hw = 0, o_thr = 0, o_off = 0, o_ch = 0, o_hu = 0, o_hv = 0;
+};
+
+static void build_plan(const fq_graph *G, const Geom &g, PlanBlob &pb)
 {
     const int N = G->N;
-    std::string key = geom_key(g);
-    key.append(reinterpret_cast<const char *>(G->W.data()), G->W.size() * sizeof(double));
-    auto note_user = [&](PlanEntry &e) {
-        e.last_use = ++G->use_clock;
-        if (std::find(e.users.begin(), e.users.end(), stream) == e.users.end()) {
-            e.users.push_back(stream);
-            e.foreign |= !is_own_stream(stream);
-        }
-    };
-    auto it = G->plans.find(key);
-    if (it != G->plans.end()) {
-        PlanEntry &e = it->second;
-        note_user(e);
-        if (e.ready && e.stream != stream) {
-            const hipError_t q = hipEventQuery(e.ready);
-            if (q == hipErrorNotReady) {
-                hipGetLastError();
-                FQ_HIP(hipStreamWaitEvent(stream, e.ready, 0));
-            } else if (q != hipSuccess) {
-                return set_err(FQ_ERR_DEVICE, "plan upload failed: %s", hipGetErrorString(q));
-            }
-        }
-        *out = e.dev;
-        return FQ_OK;
-    }
     int ti[FQ_MAXN], hi[FQ_MAXN];
     for (int p = 0; p < N; ++p)
         ti[p] = hi[p] = -1;
@@ -553,7 +529,7 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream
         }
         thr[t] = base;
     }
-    PlanDev dev{};
+    PlanDev &dev = pb.dev;
     for (int j = 0; j < 16; ++j)
         dev.rr[j] = 0;
     for (int j = 0; j < A; ++j) {
@@ -579,30 +555,86 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream
     off[g.Q] = (int)ch.size();
 
     // blob layout
-    size_t o_cH = 0;
-    size_t o_om = align16(o_cH + sizeof(double) * g.Q);
-    size_t o_cw = align16(o_om + sizeof(double) * omega.size());
-    size_t o_hw = align16(o_cw + sizeof(double) * std::max<size_t>(cw.size(), 1));
-    size_t o_thr = align16(o_hw + sizeof(double) * std::max<size_t>(hh_w.size(), 1));
-    size_t o_off = align16(o_thr + sizeof(double) * thr.size());
-    size_t o_ch = align16(o_off + sizeof(int) * off.size());
-    size_t o_hu = align16(o_ch + sizeof(int) * std::max<size_t>(ch.size(), 1));
-    size_t o_hv = align16(o_hu + sizeof(int) * std::max<size_t>(hh_u.size(), 1));
-    size_t total = align16(o_hv + sizeof(int) * std::max<size_t>(hh_v.size(), 1));
-    std::vector<char> blob(total, 0);
+    pb.o_cH = 0;
+    pb.o_om = align16(pb.o_cH + sizeof(double) * g.Q);
+    pb.o_cw = align16(pb.o_om + sizeof(double) * omega.size());
+    pb.o_hw = align16(pb.o_cw + sizeof(double) * std::max<size_t>(cw.size(), 1));
+    pb.o_thr = align16(pb.o_hw + sizeof(double) * std::max<size_t>(hh_w.size(), 1));
+    pb.o_off = align16(pb.o_thr + sizeof(double) * thr.size());
+    pb.o_ch = align16(pb.o_off + sizeof(int) * off.size());
+    pb.o_hu = align16(pb.o_ch + sizeof(int) * std::max<size_t>(ch.size(), 1));
+    pb.o_hv = align16(pb.o_hu + sizeof(int) * std::max<size_t>(hh_u.size(), 1));
+    const size_t total = align16(pb.o_hv + sizeof(int) * std::max<size_t>(hh_v.size(), 1));
+    pb.bytes.assign(total, 0);
     auto put = [&](size_t o, const void *src, size_t n) {
         if (n)
-            memcpy(blob.data() + o, src, n);
+            memcpy(pb.bytes.data() + o, src, n);
     };
-    put(o_cH, cH.data(), sizeof(double) * g.Q);
-    put(o_om, omega.data(), sizeof(double) * omega.size());
-    put(o_cw, cw.data(), sizeof(double) * cw.size());
-    put(o_hw, hh_w.data(), sizeof(double) * hh_w.size());
-    put(o_thr, thr.data(), sizeof(double) * thr.size());
-    put(o_off, off.data(), sizeof(int) * off.size());
-    put(o_ch, ch.data(), sizeof(int) * ch.size());
-    put(o_hu, hh_u.data(), sizeof(int) * hh_u.size());
-    put(o_hv, hh_v.data(), sizeof(int) * hh_v.size());
+    put(pb.o_cH, cH.data(), sizeof(double) * g.Q);
+    put(pb.o_om, omega.data(), sizeof(double) * omega.size());
+    put(pb.o_cw, cw.data(), sizeof(double) * cw.size());
+    put(pb.o_hw, hh_w.data(), sizeof(double) * hh_w.size());
+    put(pb.o_thr, thr.data(), sizeof(double) * thr.size());
+    put(pb.o_off, off.data(), sizeof(int) * off.size());
+    put(pb.o_ch, ch.data(), sizeof(int) * ch.size());
+    put(pb.o_hu, hh_u.data(), sizeof(int) * hh_u.size());
+    put(pb.o_hv, hh_v.data(), sizeof(int) * hh_v.size());
+}
+
+// The plan of a blob that lives at base (device memory).
+static PlanDev place_plan(const PlanBlob &pb, char *base)
+{
+    PlanDev dev = pb.dev;
+    dev.cH = (const double *)(base + pb.o_cH);
+    dev.omega = (const double *)(base + pb.o_om);
+    dev.csr_w = (const double *)(base + pb.o_cw);
+    dev.hh_w = (const double *)(base + pb.o_hw);
+    dev.thr = (const double *)(base + pb.o_thr);
+    dev.csr_off = (const int *)(base + pb.o_off);
+    dev.csr_h = (const int *)(base + pb.o_ch);
+    dev.hh_u = (const int *)(base + pb.o_hu);
+    dev.hh_v = (const int *)(base + pb.o_hv);
+    return dev;
+}
+
+// Device cut plan of graph G for tile geometry g, cached per (geometry, W
+// content).  A new plan is uploaded asynchronously on `stream`; a cached plan
+// whose upload was queued on another stream makes `stream` wait for it.
+// With FQ_PLAN_CACHE plans cached, the least recently used one whose buffers
+// are large enough is replaced: `stream` first waits (on the device, through
+// an event) for the work queued so far on every stream that plan was handed
+// to, so no launch still reading it sees the new contents, and the host
+// blocks on nothing but that plan's own earlier upload.
+static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream)
+{
+    std::string key = geom_key(g);
+    key.append(reinterpret_cast<const char *>(G->W.data()), G->W.size() * sizeof(double));
+    auto note_user = [&](PlanEntry &e) {
+        e.last_use = ++G->use_clock;
+        if (std::find(e.users.begin(), e.users.end(), stream) == e.users.end()) {
+            e.users.push_back(stream);
+            e.foreign |= !is_own_stream(stream);
+        }
+    };
+    auto it = G->plans.find(key);
+    if (it != G->plans.end()) {
+        PlanEntry &e = it->second;
+        note_user(e);
+        if (e.ready && e.stream != stream) {
+            const hipError_t q = hipEventQuery(e.ready);
+            if (q == hipErrorNotReady) {
+                hipGetLastError();
+                FQ_HIP(hipStreamWaitEvent(stream, e.ready, 0));
+            } else if (q != hipSuccess) {
+                return set_err(FQ_ERR_DEVICE, "plan upload failed: %s", hipGetErrorString(q));
+            }
+        }
+        *out = e.dev;
+        return FQ_OK;
+    }
+    PlanBlob pb;
+    build_plan(G, g, pb);
+    const size_t total = pb.bytes.size();
     PlanEntry pe;
     bool reused = false;
     if (G->plans.size() >= FQ_PLAN_CACHE) {
@@ -652,7 +684,7 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream
             return set_err(FQ_ERR_OOM, "plan allocation of %zu bytes failed", cap);
         }
     }
-    memcpy(pe.staging, blob.data(), total);
+    memcpy(pe.staging, pb.bytes.data(), total);
     pe.stream = stream;
     if (hipMemcpyAsync(pe.dmem, pe.staging, total, hipMemcpyHostToDevice, stream) != hipSuccess ||
         hipEventRecord(pe.ready, stream) != hipSuccess) {
@@ -661,16 +693,7 @@ static int get_plan(fq_graph *G, const Geom &g, PlanDev *out, hipStream_t stream
         plan_free(pe);
         return set_err(FQ_ERR_DEVICE, "plan upload failed");
     }
-    char *b = (char *)pe.dmem;
-    dev.cH = (const double *)(b + o_cH);
-    dev.omega = (const double *)(b + o_om);
-    dev.csr_w = (const double *)(b + o_cw);
-    dev.hh_w = (const double *)(b + o_hw);
-    dev.thr = (const double *)(b + o_thr);
-    dev.csr_off = (const int *)(b + o_off);
-    dev.csr_h = (const int *)(b + o_ch);
-    dev.hh_u = (const int *)(b + o_hu);
-    dev.hh_v = (const int *)(b + o_hv);
+    const PlanDev dev = place_plan(pb, (char *)pe.dmem);
     pe.dev = dev;
     pe.users.clear();
     pe.foreign = false;
@@ -1548,6 +1571,89 @@ extern "C" int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *
         return rc;
     double h[2];
     FQ_HIP(hipMemcpyAsync(h, st->d_out2, sizeof(h), hipMemcpyDeviceToHost, st->stream));
+    FQ_HIP(hipStreamSynchronize(st->stream));
+    *expect = h[0];
+    if (norm2)
+        *norm2 = h[1];
+    return FQ_OK;
+}
+
+// ---- allocation-free <C> ----------------------------------------------------------------
+//
+// fq_expect_cut_static runs the kernels of fq_expect_cut with their scratch
+// (per-tile partial sums, the cut plan, the two-double result) in `scratch`,
+// a fixed device array of this library, instead of memory it allocates: for
+// callers that cannot allocate device memory (code under HIP graph capture,
+// serving loops).  Each GPU holds its own copy until the process ends; every
+// call on one GPU uses that copy, so calls that overlap on one GPU race.
+
+#define FQ_STATIC_MAX_N 30
+#define FQ_STATIC_TILE_Q 12
+// 2 doubles per 2^12-amplitude tile at the largest N (4 MiB), the cut plan
+// of a 12-qubit tile (under 16 KiB at N=30 on a complete graph), the result
+#define FQ_STATIC_PART_BYTES ((size_t)16 << (FQ_STATIC_MAX_N - FQ_STATIC_TILE_Q))
+#define FQ_STATIC_PLAN_BYTES ((size_t)64 << 10)
+#define FQ_STATIC_BYTES (FQ_STATIC_PART_BYTES + FQ_STATIC_PLAN_BYTES + 256)
+
+static __device__ __align__(256) unsigned char scratch[FQ_STATIC_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char *> g_scratch_ptr;  // device address of scratch, per device
+
+// Device address of this GPU's scratch: hipGetSymbolAddress once per device,
+// with that device current.
+static int scratch_address(unsigned char **out)
+{
+    int dev = 0;
+    FQ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev)
+        g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(FQ_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FQ_OK;
+}
+
+extern "C" int fq_expect_cut_static(fq_state *st, fq_graph *g, double *expect, double *norm2)
+{
+    if (int rc = check_state(st))
+        return rc;
+    if (int rc = graph_matches(st, g))
+        return rc;
+    if (!expect)
+        return set_err(FQ_ERR_ARG, "null output pointer");
+    if (st->N > FQ_STATIC_MAX_N)
+        return set_err(FQ_ERR_ARG, "N=%d exceeds the static scratch of fq_expect_cut_static (N <= %d)", st->N,
+                       FQ_STATIC_MAX_N);
+    unsigned char *base = nullptr;
+    if (int rc = scratch_address(&base))
+        return rc;
+    double *part = (double *)base;
+    char *plan_mem = (char *)(base + FQ_STATIC_PART_BYTES);
+    double *out2 = (double *)(base + FQ_STATIC_PART_BYTES + FQ_STATIC_PLAN_BYTES);
+    const int Q = std::min(FQ_STATIC_TILE_Q, st->N);
+    const int a_log = std::max(1, std::min(4, Q));
+    Geom geom = make_geom(st->N, low_tile(Q), a_log);
+    TileArgs a;
+    geom_to_args(geom, a);
+    PlanBlob pb;
+    build_plan(g, geom, pb);
+    if (pb.bytes.size() > FQ_STATIC_PLAN_BYTES)
+        return set_err(FQ_ERR_ARG, "cut plan of %zu bytes exceeds the static scratch (%zu)", pb.bytes.size(),
+                       (size_t)FQ_STATIC_PLAN_BYTES);
+    a.plan = place_plan(pb, plan_mem);
+    // the previous call on this stream may still read the plan: stream order
+    // keeps this upload behind it
+    FQ_HIP(hipMemcpyAsync(plan_mem, pb.bytes.data(), pb.bytes.size(), hipMemcpyHostToDevice, st->stream));
+    FQ_HIP(launch_expect(st->dtype, a_log, st->sv, a, part, out2, st->stream));
+    double h[2];
+    FQ_HIP(hipMemcpyAsync(h, out2, sizeof(h), hipMemcpyDeviceToHost, st->stream));
     FQ_HIP(hipStreamSynchronize(st->stream));
     *expect = h[0];
     if (norm2)

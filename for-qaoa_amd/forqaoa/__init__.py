@@ -194,6 +194,14 @@ class State:
         check(lib().fq_expect_cut(self.handle, graph.handle, ctypes.byref(e), ctypes.byref(n)))
         return e.value, n.value
 
+    def expect_cut_static(self, graph):
+        """expect_cut without device allocation (fq_expect_cut_static: N <= 30,
+        scratch in a fixed device array shared by every call on the GPU)."""
+        e = ctypes.c_double()
+        n = ctypes.c_double()
+        check(lib().fq_expect_cut_static(self.handle, graph.handle, ctypes.byref(e), ctypes.byref(n)))
+        return e.value, n.value
+
     def topk(self, k):
         idx = np.zeros(k, dtype=np.uint64)
         prob = np.zeros(k, dtype=np.float64)

@@ -77,6 +77,7 @@ SIGNATURES = {
     "fq_swap_conti": (_i, [_vp, _i, _i, _i]),
     "fq_run": (_i, [_vp, _vp, _i, _dp, _dp, _i]),
     "fq_expect_cut": (_i, [_vp, _vp, _dp, _dp]),
+    "fq_expect_cut_static": (_i, [_vp, _vp, _dp, _dp]),
     "fq_topk": (_i, [_vp, _i, _up, _dp]),
     "fq_qubit_map": (_i, [_vp, _ip]),
     "fq_state_copy_out": (_i, [_vp, _vp, _sz, _i]),

@@ -142,6 +142,14 @@ int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, const double 
 /* <C> = sum_b |psi_b|^2 Cut(b), W in physical labels; FP64 accumulation,
  * deterministic order.  norm2 (may be NULL) receives sum |psi|^2.  Waits. */
 int fq_expect_cut(fq_state *st, fq_graph *g, double *expect, double *norm2);
+/* fq_expect_cut without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that must not call hipMalloc).  Its kernels keep
+ * their scratch (partial sums, cut plan, result) in a fixed 4.3 MB device
+ * array of the library, one per GPU, held until the process ends; states of
+ * N <= 30 only (FQ_ERR_ARG beyond).  Calls that overlap on one GPU share
+ * that array and race.  Waits. */
+int fq_expect_cut_static(fq_state *st, fq_graph *g, double *expect, double *norm2);
 /* The k <= 16 most probable basis states, best first (ties: smaller physical
  * index); idx receives LOGICAL indices (via the qubit map).  Waits. */
 int fq_topk(fq_state *st, int k, uint64_t *idx, double *prob);
