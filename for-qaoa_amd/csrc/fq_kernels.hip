@@ -124,6 +124,18 @@ __device__ __forceinline__ uint64_t tile_at(const TileArgs &a, uint64_t ntiles, 
     return k * G + b;
 }
 
+// Hides the value of an LDS base index from the compiler between a phase's
+// reads and its write-back, so the write-back recomputes its 2^A_LOG
+// addresses (one add each) instead of keeping them live in VGPRs through the
+// phase's butterflies.  Used where the VGPRs run short: the complex64 passes
+// with the cost phase (with the addresses kept, the boundary pass spilled 36
+// bytes per lane, three of them prefetched amplitudes whose spill waited for
+// every load in flight).
+__device__ __forceinline__ void lds_opaque(int &x)
+{
+    asm volatile("" : "+v"(x));
+}
+
 // RX on the gate positions of a tile held in the natural layout (register j
 // of thread t = tile element t + j * 2^nthr), returned in the same layout:
 // register bits in registers, thread bits through the LDS remapping phases.
@@ -145,7 +157,7 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
     lds_barrier();
 #pragma unroll 1
     for (int g = 0; g < a.ngroups; ++g) {
-        const int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
+        int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
         const unsigned gm = group_mask(a, g);
         int pr[A_LOG > 0 ? A_LOG : 1];
         unsigned g2 = gm;
@@ -167,6 +179,7 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
 #pragma unroll
         for (int j = 0; j < A; ++j)
             v[j] = lds[eg + ej[j]];
+        lds_opaque(eg);
         rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
 #pragma unroll
         for (int j = 0; j < A; ++j)
@@ -394,7 +407,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                 // phase's addresses out of the tile loop and spill
 #pragma unroll 1
                 for (int g = 0; g < a.ngroups; ++g) {
-                    const int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
+                    int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
                     const unsigned gm = group_mask(a, g);
                     int pr[A_LOG > 0 ? A_LOG : 1];
                     unsigned g2 = gm;
@@ -416,6 +429,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
 #pragma unroll
                     for (int j = 0; j < A; ++j)
                         v[j] = lds[eg + ej[j]];
+                    if (COST && sizeof(R) == 4)
+                        lds_opaque(eg);
                     rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
                     if (direct && g == a.ngroups - 1)
                         break;
