@@ -126,6 +126,20 @@ void fq::rx_coeffs(double beta, int dtype, double *c, double *s)
     *s = std::copysign((double)sb, sn);
 }
 
+void fq::rx_tform(double beta, int dtype, bool flip, double *t, double *sigma_re, double *sigma_im)
+{
+    double sn, cs;
+    sincos(beta / 2, &sn, &cs);
+    const bool f = flip && std::fabs(sn) > std::fabs(cs);
+    double tt = f ? -cs / sn : sn / cs;
+    if (dtype == FQ_C64)
+        tt = (double)(float)tt;
+    *t = tt;
+    const double mag = 1.0 / std::sqrt(1.0 + tt * tt);
+    *sigma_re = f ? 0.0 : std::copysign(mag, cs);
+    *sigma_im = f ? -std::copysign(mag, sn) : 0.0;
+}
+
 // RX(beta) as applied with rx_coeffs is sqrt(c^2 + s^2) times a unitary, a
 // global factor per gate.  A P-layer evaluation gives every qubit one RX per
 // layer, so its state is prod_p (c_p^2 + s_p^2)^(N/2) times the state of a
@@ -222,6 +236,12 @@ struct fq_state {
     hipStream_t own = nullptr, stream = nullptr;
     int qmap[FQ_MAXN];
     double init_amp = 0;
+    // product of the RX factors of tile passes that left theirs to a later
+    // pass (run_layer); 1 between entry points
+    double pending_re = 1.0, pending_im = 0.0;
+    // inside fq_run: every RX applies one angle to all qubits of an
+    // X^(x)N-symmetric state, so rx_tform may drop its X (see rx_tform)
+    bool in_run = false;
     double *d_part = nullptr;  // expect partials
     size_t part_cap = 0;
     double *d_out2 = nullptr;
@@ -384,6 +404,7 @@ static uint32_t tile_order()
 static void geom_to_args(const Geom &g, TileArgs &a)
 {
     memset(&a, 0, sizeof(a));
+    a.scale_re = 1.0;
     a.order = tile_order();
     a.N = g.N;
     a.Q = g.Q;
@@ -964,6 +985,7 @@ static std::vector<int> low_tile(int Q)
 // Cost-phase launch arguments: angle and launch-control amplitude.
 static void set_cost(TileArgs &a, double gamma, double init_amp)
 {
+    // a.scale (the pass's real RX factor, 1 without gates) is set by the caller
     a.gamma = gamma;
     a.init_amp = init_amp;
     a.has_rr = 0;
@@ -1061,10 +1083,16 @@ static int pass_args(int N, int dtype, const std::vector<int> &tile, const std::
 // One tile pass with RX on gate positions (and optionally the cost phase).
 // pre_beta (with cost, not first, not a mirror pass): RX(*pre_beta) on the
 // same positions before the cost phase -- the pass that ends one layer and
-// opens the next.
+// opens the next.  The kernel applies each RX as the one-FMA butterfly of
+// rx_tform, i.e. sigma^-1 RX per gate; the pass's factor sigma^(#gates) (times
+// sigma_pre^(#gates)) and any factor earlier passes left pending go into the
+// cost phase of a cost pass; a pass without it multiplies them in before its
+// stores, unless defer is set: then they stay pending for the next pass of the
+// same fq_run (a layer's RX factors grow the amplitudes by at most 2^(N/2)
+// before a cost pass absorbs them).
 static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vector<int> &gates,
                      double beta, fq_graph *g, double gamma, bool cost, bool first, int kind,
-                     int z2 = Z2_OFF, const double *pre_beta = nullptr)
+                     int z2 = Z2_OFF, const double *pre_beta = nullptr, bool defer = false)
 {
     if (pre_beta && (!cost || first || z2 >= Z2_MIRROR))
         return set_err(FQ_ERR_ARG, "internal: a layer-boundary pass needs the cost phase of a later layer");
@@ -1073,9 +1101,30 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     if (int rc = pass_args(st->N, st->dtype, tile, gates, cost, z2, a, geom))
         return rc;
     const int a_log = geom.a_log;
-    rx_coeffs(beta, st->dtype, &a.rx_c, &a.rx_s);
+    // the pass's factor: sigma^(#gates) per RX set, times what is pending
+    double sr = st->pending_re, si = st->pending_im;
+    auto absorb = [&](double b, double *t) {
+        double gr, gi;
+        rx_tform(b, st->dtype, st->in_run, t, &gr, &gi);
+        for (size_t k = 0; k < gates.size(); ++k) {
+            const double nr = sr * gr - si * gi;
+            si = sr * gi + si * gr;
+            sr = nr;
+        }
+    };
+    absorb(beta, &a.rx_t);
     if (pre_beta)
-        rx_coeffs(*pre_beta, st->dtype, &a.pre_c, &a.pre_s);
+        absorb(*pre_beta, &a.pre_t);
+    if (cost || !defer) {
+        a.scale_re = sr;
+        a.scale_im = si;
+        a.apply_scale = !cost;
+        st->pending_re = 1.0;
+        st->pending_im = 0.0;
+    } else {
+        st->pending_re = sr;
+        st->pending_im = si;
+    }
     if (cost) {
         if (int rc = get_plan(g, geom, &a.plan, st->stream))
             return rc;
@@ -1236,7 +1285,26 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype, int bound
 }
 
 // Split arbitrary RX targets into tile passes (targets below C: one chunk pass).
+static int rx_targets_once(fq_state *st, std::vector<int> targets, double beta);
+
 static int rx_targets(fq_state *st, std::vector<int> targets, double beta)
+{
+    if (!st->in_run) {
+        // outside fq_run the butterfly keeps its X, so |t| = |tan(beta/2)|:
+        // beyond 2.5 (an amplitude growth above 2.7 per gate before the pass's
+        // factor is applied) RX(beta) runs as two passes of RX(beta/2)
+        double sn, cs;
+        sincos(beta / 2, &sn, &cs);
+        if (std::fabs(sn) > 2.5 * std::fabs(cs)) {
+            if (int rc = rx_targets_once(st, targets, beta / 2))
+                return rc;
+            return rx_targets_once(st, targets, beta / 2);
+        }
+    }
+    return rx_targets_once(st, targets, beta);
+}
+
+static int rx_targets_once(fq_state *st, std::vector<int> targets, double beta)
 {
     const int N = st->N;
     const int Ceff = std::min(st->C, N);
@@ -1420,37 +1488,43 @@ static int run_layer(fq_state *st, fq_graph *g, const LayerPlan &lp, int p, int 
             return (int)Z2_OFF;
         return i + 1 < m ? (int)Z2_LOWER : last && expand ? (int)Z2_EXPAND : (int)Z2_MIRROR;
     };
-    auto plain = [&](int i) {
-        return tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, 0.0, false, false, FQ_K_FUSED, mode(i));
-    };
+    // the passes of this layer in launch order: (pass, kind) with kind 0 the
+    // layer-0 launch-control pass, 1 a cost pass of layer order, 2 a plain
+    // pass, 3 the boundary pass that opens layer p+1
+    std::vector<std::pair<int, int>> seq;
     if (lp.end1 < 0) {
-        for (int i = 0; i < m; ++i) {
-            const int rc = i == 0 ? tile_pass(st, ps[0].tile, ps[0].gates, betas[p], g, gammas[p], true, p == 0,
-                                              FQ_K_FUSED, mode(0))
-                                  : plain(i);
-            if (rc)
-                return rc;
-        }
-        return FQ_OK;
+        for (int i = 0; i < m; ++i)
+            seq.push_back({i, i == 0 ? (p == 0 ? 0 : 1) : 2});
+    } else {
+        const int e = p % 2 == 0 ? lp.end1 : 0;   // the pass that ends layer p (the other one opened it)
+        if (p == 0)
+            seq.push_back({0, 0});
+        if (last)
+            seq.push_back({e, 2});
+        for (int i = 1; i < m; ++i)
+            if (i != lp.end1)
+                seq.push_back({i, 2});
+        if (!last)
+            seq.push_back({e, 3});
     }
-    const int e = p % 2 == 0 ? lp.end1 : 0;   // the pass that ends layer p (the other one opened it)
-    if (p == 0)
-        if (int rc = tile_pass(st, ps[0].tile, ps[0].gates, betas[0], g, gammas[0], true, true, FQ_K_FUSED,
-                               mode(0)))
-            return rc;
-    if (last)
-        if (int rc = plain(e))
-            return rc;
-    for (int i = 1; i < m; ++i) {
-        if (i == lp.end1)
-            continue;
-        if (int rc = plain(i))
+    for (size_t k = 0; k < seq.size(); ++k) {
+        const int i = seq[k].first, kind = seq[k].second;
+        // plain passes leave their RX factor to the next cost pass, except the
+        // run's very last pass, which leaves the state exact
+        const bool defer = !(last && k + 1 == seq.size());
+        int rc;
+        if (kind == 2)
+            rc = tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, 0.0, false, false, FQ_K_FUSED, mode(i),
+                           nullptr, defer);
+        else if (kind == 3)
+            rc = tile_pass(st, ps[i].tile, ps[i].gates, betas[p + 1], g, gammas[p + 1], true, false, FQ_K_FUSED,
+                           mode(i), &betas[p]);
+        else
+            rc = tile_pass(st, ps[i].tile, ps[i].gates, betas[p], g, gammas[p], true, kind == 0, FQ_K_FUSED,
+                           mode(i));
+        if (rc)
             return rc;
     }
-    if (!last)
-        if (int rc = tile_pass(st, ps[e].tile, ps[e].gates, betas[p + 1], g, gammas[p + 1], true, false,
-                               FQ_K_FUSED, mode(e), &betas[p]))
-            return rc;
     return FQ_OK;
 }
 
@@ -1476,13 +1550,13 @@ extern "C" int fq_run(fq_state *st, fq_graph *g, int P, const double *gammas, co
         if (!std::isfinite(gammas[p]) || !std::isfinite(betas[p]))
             return set_err(FQ_ERR_ARG, "angles must be finite");
     const int N = st->N;
-    // launch-control amplitude of this evaluation (restored on return)
-    struct AmpScale {
+    st->pending_re = 1.0;
+    st->pending_im = 0.0;
+    struct InRun {
         fq_state *st;
-        double saved;
-        ~AmpScale() { st->init_amp = saved; }
-    } amp_guard{st, st->init_amp};
-    st->init_amp *= rx_norm_scale(N, P, betas, st->dtype);
+        ~InRun() { st->in_run = false; }
+    } in_run_guard{st};
+    st->in_run = true;
     if (schedule == FQ_SCHED_REFERENCE) {
         // src/Weighted.cpp:97-137
         std::vector<int> sizes;
@@ -2470,7 +2544,9 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         const int L = h.layer;
         rx_coeffs(h.betas[L], sb->dtype, &sp.rx_c, &sp.rx_s);
         sp.gamma = h.gammas[L];
-        sp.init_amp = sb->init_amp * h.amp_scale;
+        // the packed kernels apply RX with rx_coeffs (norm factor amp_scale);
+        // per-slot slots run fq_run's passes, whose RX factors are exact
+        sp.init_amp = sb->init_amp * (sb->per_slot ? 1.0 : h.amp_scale);
         sp.active = 1;
         sp.cost_first = L == 0;
         sp.want_expect = L + 1 == h.P;
@@ -2496,6 +2572,11 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
             SlotHost &h = sb->slots[k];
             fq_state *v = sb->views[k];
             v->init_amp = sp.init_amp;
+            if (h.layer == 0) {
+                v->pending_re = 1.0;
+                v->pending_im = 0.0;
+            }
+            v->in_run = true;   // a slot runs fq_run's layers on a QAOA state
             if (int rc = run_layer(v, h.g, sb->slot_plan, h.layer, h.P, h.gammas.data(), h.betas.data(), sb->z2,
                                    false))
                 return rc;

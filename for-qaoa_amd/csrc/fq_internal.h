@@ -69,6 +69,11 @@ struct TileArgs {
     uint32_t direct_store;   // last LDS phase holds no lane bit: store from its layout
     double rx_c, rx_s;       // cos(beta/2), sin(beta/2) (host-computed, src/rx.cu:121)
     double pre_c, pre_s;     // layer-boundary pass: RX of the previous layer, applied first
+    // k_tile applies RX up to a factor as the butterfly [[1, -it], [-it, 1]]
+    // (rx_tform), one FMA per output component
+    double rx_t, pre_t;
+    double scale_re, scale_im;  // factor of the pass: folded into the cost phase,
+    int apply_scale;            // or (apply_scale) multiplied in before the stores
     double gamma, init_amp;  // cost phase angle, launch-control amplitude
     int has_rr;              // any register-register edge in the plan
     PlanDev plan;
@@ -173,5 +178,16 @@ size_t tile_lds_bytes(int dtype, int Q);
 // the state by exactly c^2 + s^2, so that error compounds over every qubit
 // and layer into a drift of N*P*(c^2 + s^2 - 1) in <C>.
 void rx_coeffs(double beta, int dtype, double *c, double *s);
+// RX(beta) as sigma * B(t), B(t) = [[1, -it], [-it, 1]] with t rounded to the
+// amplitude type of dtype and sigma (FP64, from the rounded t) making
+// sigma * B(t) exactly unitary: RX of an angle within one rounding of beta.
+// t = tan(beta/2), sigma = cos(beta/2)-like, real.  With flip and
+// |sin(beta/2)| > |cos(beta/2)| instead RX(beta) = -i sin(beta/2) X B(t') with
+// t' = tan((beta - pi)/2) = -cot(beta/2): the X is dropped and sigma is
+// -i/sqrt(1 + t'^2) (signed).  Dropping X is exact only where the caller
+// applies RX(beta) to every qubit of an X^(x)N-symmetric state (a QAOA layer:
+// the dropped X's form X^(x)N, which commutes with every layer and leaves the
+// state unchanged).  |t| <= 1 with flip; without it |t| = |tan(beta/2)|.
+void rx_tform(double beta, int dtype, bool flip, double *t, double *sigma_re, double *sigma_im);
 
 }  // namespace fq

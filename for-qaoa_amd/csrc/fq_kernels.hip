@@ -143,11 +143,11 @@ __device__ __forceinline__ void lds_opaque(int &x)
 // none), so the first write here needs no barrier.
 template <typename T, typename R, int A_LOG>
 __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &a, const int *etp, int nat,
-                                            int nthr, R c, R s)
+                                            int nthr, R t)
 {
     constexpr int A = 1 << A_LOG;
     extern __shared__ __align__(16) unsigned char fq_smem[];
-    rx_registers<T, R, A_LOG>(v, a.gate_mask >> nthr, c, s);
+    rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, t);
     if (a.ngroups == 0)
         return;
     T *lds = reinterpret_cast<T *>(fq_smem);
@@ -180,7 +180,7 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
         for (int j = 0; j < A; ++j)
             v[j] = lds[eg + ej[j]];
         lds_opaque(eg);
-        rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
+        rx_registers_t<T, R, A_LOG>(v, group_gates(a, g), t);
 #pragma unroll
         for (int j = 0; j < A; ++j)
             lds[eg + ej[j]] = v[j];
@@ -197,8 +197,13 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
 // QT > 0 fixes the tile width at compile time (the fused passes' full-width
 // tiles); QT == 0 reads it from the arguments.
 // PRE: the pass that ends one layer and opens the next applies RX of the
-// previous layer (pre_c, pre_s) on the same gate positions before the cost
+// previous layer (pre_t, pre_form) on the same gate positions before the cost
 // phase (COST, not FIRST, MIRROR 0 only).
+// RX is applied as the one-FMA butterfly of rx_tform, i.e. up to a factor per
+// gate; the host passes the product of those factors (and of any earlier
+// passes that left theirs pending) as a.scale_re + i a.scale_im, which a cost
+// pass folds into its FP64 phase factor and a pass without the cost phase
+// multiplies in before its stores when a.apply_scale is set.
 template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false>
 __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a,
                                                             const double2 *__restrict__ ptab)
@@ -321,7 +326,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         const uint64_t next = tile_at(a, ntiles, ++kt);
 
         if (PRE)
-            mix_natural<T, R, A_LOG>(v, a, etp, nat, nthr, (R)a.pre_c, (R)a.pre_s);
+            mix_natural<T, R, A_LOG>(v, a, etp, nat, nthr, (R)a.pre_t);
 
         // Cost phase first (few registers live), then the prefetch.
         // exp(i*gamma*s/2), s = 2*Cut - W, of register j is the product of the
@@ -343,9 +348,13 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                     y = on ? ny : y;
                 }
             }
-            if (FIRST) {
-                x *= a.init_amp;
-                y *= a.init_amp;
+            // the pass's RX factor (and the launch-control amplitude)
+            {
+                const double sr = FIRST ? a.init_amp * a.scale_re : a.scale_re;
+                const double si = FIRST ? a.init_amp * a.scale_im : a.scale_im;
+                const double nx = x * sr - y * si;
+                y = x * si + y * sr;
+                x = nx;
             }
             const R c0 = (R)x, s0 = (R)y;
 #pragma unroll
@@ -393,8 +402,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         }
 
         if (GATES) {
-            const R c = (R)a.rx_c, s = (R)a.rx_s;
-            rx_registers<T, R, A_LOG>(v, a.gate_mask >> nthr, c, s);
+            const R rt = (R)a.rx_t;
+            rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, rt);
             if (a.ngroups > 0) {
                 T *lds = reinterpret_cast<T *>(fq_smem);
                 // own slots: the previous tile's final read used the same
@@ -431,7 +440,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                         v[j] = lds[eg + ej[j]];
                     if (COST && sizeof(R) == 4)
                         lds_opaque(eg);
-                    rx_registers<T, R, A_LOG>(v, group_gates(a, g), c, s);
+                    rx_registers_t<T, R, A_LOG>(v, group_gates(a, g), rt);
                     if (direct && g == a.ngroups - 1)
                         break;
 #pragma unroll
@@ -449,6 +458,12 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                         v[j] = lds[nat + lds_pad(j << nthr)];
                 }
             }
+        }
+
+        if (!COST && a.apply_scale) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                cmul_by(v[j], (R)a.scale_re, (R)a.scale_im);
         }
 
         if (direct) {

@@ -295,9 +295,23 @@ def one_at_a_time(stream, requests, dtype="c64", schedule="reference", C=None):
 
 
 def latency_stats(done):
+    """p50 / p99 / mean latency (arrival to result) in ms, overall and per
+    qubit-count band (N 16-20, 21-25, 26-28, 29-30), with the time from
+    arrival to admission (queueing) beside it."""
     lat = np.array([r.done - r.arrival for r in done])
-    return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
-                mean_ms=float(lat.mean() * 1e3))
+    wait = np.array([r.start - r.arrival for r in done])
+    out = dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+               mean_ms=float(lat.mean() * 1e3), queue_p99_ms=float(np.percentile(wait, 99) * 1e3))
+    bands = {}
+    for lo, hi in ((0, 20), (21, 25), (26, 28), (29, 40)):
+        sel = np.array([lo <= r.N <= hi for r in done])
+        if sel.any():
+            bands[f"N{max(lo, min(r.N for r in done))}-{min(hi, max(r.N for r in done))}"] = dict(
+                n=int(sel.sum()), p50_ms=float(np.percentile(lat[sel], 50) * 1e3),
+                p99_ms=float(np.percentile(lat[sel], 99) * 1e3),
+                queue_p99_ms=float(np.percentile(wait[sel], 99) * 1e3))
+    out["by_n"] = bands
+    return out
 
 
 def bytes_floor(stream, requests, dtype="c64", schedule="fused"):
