@@ -14,7 +14,8 @@ launch-control layer through layer 5, inputs and state resident in HBM.
 value = 5 * steps / seconds.
 
 Also reported (extra): the same evaluation on the 3-regular graph, in
-complex64, with the reference schedule (cost, RX chunks, transpositions), and
+complex64, with the reference schedule (cost, RX chunks, transpositions) in
+both amplitude types, and
 N=34 P=10 complex64 on G(34, 0.3) with integer weights, and the half-state
 schedule (layers on the z_{N-1} = 0 half, psi(b) = psi(~b); the full state is
 written after the last layer) for N=30 and N=34.
@@ -319,6 +320,7 @@ def main():
         for name, dt, sch, steps, warm in [("n30_reg3", "c128", "fused", 3, 1),
                                             ("n30_gnp05", "c64", "fused", 3, 1),
                                             ("n30_gnp05", "c128", "reference", 2, 1),
+                                            ("n30_gnp05", "c64", "reference", 2, 1),
                                             ("n34_gnp03_int", "c64", "fused", 1, 1),
                                             ("n30_gnp05", "c128", "z2", 3, 1),
                                             ("n30_gnp05", "c64", "z2", 3, 1),
