@@ -1126,11 +1126,32 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
         st->pending_im = si;
     }
     if (cost) {
-        if (int rc = get_plan(g, geom, &a.plan, st->stream))
+        // the complex64 boundary pass applies the cost phase in its last LDS
+        // phase's layout: thread bits = the tile bits outside that phase,
+        // register bits = the phase's bits (both in increasing order), so its
+        // phase table comes from the plan of that ordering of the same tile
+        TileArgs ca = a;
+        Geom cgeom = geom;
+        if (pre_beta && st->dtype == FQ_C64 && a.ngroups > 0) {
+            const unsigned gm = a.grp_mask[a.ngroups - 1];
+            std::vector<int> t2;
+            for (int i = 0; i < geom.Q; ++i)
+                if (!((gm >> i) & 1u))
+                    t2.push_back(geom.tpos[i]);
+            for (int i = 0; i < geom.Q; ++i)
+                if ((gm >> i) & 1u)
+                    t2.push_back(geom.tpos[i]);
+            cgeom = make_geom(st->N, t2, a_log);
+        }
+        if (int rc = get_plan(g, cgeom, &ca.plan, st->stream))
             return rc;
-        set_cost(a, gamma, st->init_amp);
-        if (int rc = prepare_phases(st, a, a_log))
+        set_cost(ca, gamma, st->init_amp);
+        if (int rc = prepare_phases(st, ca, a_log))
             return rc;
+        a.plan = ca.plan;
+        a.gamma = ca.gamma;
+        a.init_amp = ca.init_amp;
+        a.has_rr = ca.has_rr;
     }
     // a half-state pass reads and writes half the state; the expanding one
     // writes the whole state

@@ -325,14 +325,13 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     while (tile < ntiles) {
         const uint64_t next = tile_at(a, ntiles, ++kt);
 
-        if (PRE)
-            mix_natural<T, R, A_LOG>(v, a, etp, nat, nthr, (R)a.pre_t);
-
-        // Cost phase first (few registers live), then the prefetch.
-        // exp(i*gamma*s/2), s = 2*Cut - W, of register j is the product of the
-        // thread's base factor, one factor per register bit set in j and the
-        // register-register term; all factors come from the phase table.
-        if (COST) {
+        // Cost phase: exp(i*gamma*s/2), s = 2*Cut - W, of register j is the
+        // product of the thread's base factor, one factor per register bit
+        // set in j and the register-register term; all factors come from the
+        // phase table, whose thread and register bits are those of the layout
+        // the registers hold when it is applied (the natural layout, or in the
+        // merged boundary pass the last LDS phase's).
+        auto apply_cost = [&]() {
             const double2 *f_t = ptab + tile * q1;
             auto row = [&](int i) -> double2 { return ROWPF ? lane_bcast(fcur, i) : f_t[i]; };
             double x = f_thr.x, y = f_thr.y;
@@ -383,79 +382,145 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                     cmul_by(v[j], (R)f.x, (R)f.y);
                 }
             }
-        }
-
-        if (ROWPF && next < ntiles)
-            fnext = ptab[next * q1 + row_lane];
+        };
         T w[A];
-        if (!FIRST && next < ntiles) {
-            const T *p1 = sv + (tile_base(a, next) + toff);
+        // the next tile's loads (issued after the cost phase: few registers live)
+        auto prefetch = [&]() {
+            if (ROWPF && next < ntiles)
+                fnext = ptab[next * q1 + row_lane];
+            if (!FIRST && next < ntiles) {
+                const T *p1 = sv + (tile_base(a, next) + toff);
 #pragma unroll
-            for (int j = 0; j < AH; ++j)
-                w[j] = p1[roff[j]];
-            if (MIRROR) {
-                const T *pm = sv + (tile_base(a, ~next & hmask) + a.mirror_pm - toff);
+                for (int j = 0; j < AH; ++j)
+                    w[j] = p1[roff[j]];
+                if (MIRROR) {
+                    const T *pm = sv + (tile_base(a, ~next & hmask) + a.mirror_pm - toff);
 #pragma unroll
-                for (int j = AH; j < A; ++j)
-                    w[j] = pm[-(int64_t)roff[j - AH]];
-            }
-        }
-
-        if (GATES) {
-            const R rt = (R)a.rx_t;
-            rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, rt);
-            if (a.ngroups > 0) {
-                T *lds = reinterpret_cast<T *>(fq_smem);
-                // own slots: the previous tile's final read used the same
-                // slots, so no barrier is needed before this write
-#pragma unroll
-                for (int j = 0; j < A; ++j)
-                    lds[nat + lds_pad(j << nthr)] = v[j];
-                lds_barrier();
-                // rolled loop: unrolling lets the compiler hoist every
-                // phase's addresses out of the tile loop and spill
-#pragma unroll 1
-                for (int g = 0; g < a.ngroups; ++g) {
-                    int eg = g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
-                    const unsigned gm = group_mask(a, g);
-                    int pr[A_LOG > 0 ? A_LOG : 1];
-                    unsigned g2 = gm;
-#pragma unroll
-                    for (int r = 0; r < A_LOG; ++r) {
-                        pr[r] = __builtin_ctz(g2);
-                        g2 &= g2 - 1u;
-                    }
-                    int ej[A];
-#pragma unroll
-                    for (int j = 0; j < A; ++j) {
-                        int e = 0;
-#pragma unroll
-                        for (int r = 0; r < A_LOG; ++r)
-                            if ((j >> r) & 1)
-                                e |= 1 << pr[r];
-                        ej[j] = lds_pad(e);
-                    }
-#pragma unroll
-                    for (int j = 0; j < A; ++j)
-                        v[j] = lds[eg + ej[j]];
-                    if (COST && sizeof(R) == 4)
-                        lds_opaque(eg);
-                    rx_registers_t<T, R, A_LOG>(v, group_gates(a, g), rt);
-                    if (direct && g == a.ngroups - 1)
-                        break;
-#pragma unroll
-                    for (int j = 0; j < A; ++j)
-                        lds[eg + ej[j]] = v[j];
-                    lds_barrier();
+                    for (int j = AH; j < A; ++j)
+                        w[j] = pm[-(int64_t)roff[j - AH]];
                 }
-                if (direct) {
-                    // every thread has read its last-phase elements before
-                    // the next tile's first LDS writes
-                    lds_barrier();
-                } else {
+            }
+        };
+        T *lds = reinterpret_cast<T *>(fq_smem);
+        // LDS slots of LDS phase g (padded): eg for this thread, ej[j] for register j
+        int ej[A];
+        auto phase_slots = [&](int g) -> int {
+            const unsigned gm = group_mask(a, g);
+            int pr[A_LOG > 0 ? A_LOG : 1];
+            unsigned g2 = gm;
 #pragma unroll
-                    for (int j = 0; j < A; ++j)
-                        v[j] = lds[nat + lds_pad(j << nthr)];
+            for (int r = 0; r < A_LOG; ++r) {
+                pr[r] = __builtin_ctz(g2);
+                g2 &= g2 - 1u;
+            }
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                int e = 0;
+#pragma unroll
+                for (int r = 0; r < A_LOG; ++r)
+                    if ((j >> r) & 1)
+                        e |= 1 << pr[r];
+                ej[j] = lds_pad(e);
+            }
+            return g == 0 ? etp[0] : g == 1 ? etp[1] : g == 2 ? etp[2] : g == 3 ? etp[3] : etp[4];
+        };
+        // read LDS phase g into the registers and apply RX(tt) on its gated bits
+        auto phase_read_rx = [&](int g, int &eg, R tt) {
+            eg = phase_slots(g);
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                v[j] = lds[eg + ej[j]];
+            if (COST && sizeof(R) == 4)
+                lds_opaque(eg);
+            rx_registers_t<T, R, A_LOG>(v, group_gates(a, g), tt);
+        };
+        auto phase_write = [&](int eg) {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                lds[eg + ej[j]] = v[j];
+            lds_barrier();
+        };
+        auto natural_write = [&]() {
+            // own slots: the previous tile's final read used the same slots,
+            // so no barrier is needed before this write
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                lds[nat + lds_pad(j << nthr)] = v[j];
+            lds_barrier();
+        };
+        auto natural_read = [&]() {
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                v[j] = lds[nat + lds_pad(j << nthr)];
+        };
+
+        if constexpr (PRE && sizeof(R) == 4) {
+            // complex64 layer-boundary pass: RX(pre) through its LDS phases, the cost
+            // phase applied in the last phase's layout (with the phase table
+            // of that layout; the natural one without LDS phases), RX of this
+            // layer on that phase's bits while the registers still hold it,
+            // then the other phases and the natural register bits: two LDS
+            // round trips fewer than finishing RX(pre) in the natural layout
+            // and starting again.
+            const R pt = (R)a.pre_t, rt = (R)a.rx_t;
+            const int G = a.ngroups;
+            rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, pt);
+            int eg = 0;
+            if (G > 0) {
+                natural_write();
+#pragma unroll 1
+                for (int g = 0; g < G; ++g) {
+                    phase_read_rx(g, eg, pt);
+                    if (g + 1 == G)
+                        break;
+                    phase_write(eg);
+                }
+            }
+            apply_cost();
+            prefetch();
+            if (G > 0) {
+                rx_registers_t<T, R, A_LOG>(v, group_gates(a, G - 1), rt);
+                eg = phase_slots(G - 1);
+                phase_write(eg);
+#pragma unroll 1
+                for (int g = 0; g + 1 < G; ++g) {
+                    phase_read_rx(g, eg, rt);
+                    phase_write(eg);
+                }
+                natural_read();
+            }
+            rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, rt);
+        } else {
+            // (the complex128 boundary pass, at 128 VGPRs with its row
+            // prefetch, spills in the merged order: it finishes RX(pre) in
+            // the natural layout instead)
+            if (PRE)
+                mix_natural<T, R, A_LOG>(v, a, etp, nat, nthr, (R)a.pre_t);
+            if (COST)
+                apply_cost();
+            prefetch();
+            if (GATES) {
+                const R rt = (R)a.rx_t;
+                rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, rt);
+                if (a.ngroups > 0) {
+                    natural_write();
+                    // rolled loop: unrolling lets the compiler hoist every
+                    // phase's addresses out of the tile loop and spill
+                    int eg = 0;
+#pragma unroll 1
+                    for (int g = 0; g < a.ngroups; ++g) {
+                        phase_read_rx(g, eg, rt);
+                        if (direct && g == a.ngroups - 1)
+                            break;
+                        phase_write(eg);
+                    }
+                    if (direct) {
+                        // every thread has read its last-phase elements before
+                        // the next tile's first LDS writes
+                        lds_barrier();
+                    } else {
+                        natural_read();
+                    }
                 }
             }
         }
