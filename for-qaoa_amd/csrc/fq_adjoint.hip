@@ -92,27 +92,22 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     const uint64_t ntiles = run_tiles(a);
     const uint64_t hmask = (1ull << a.nh) - 1ull;
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
-    uint64_t roff[A];
-#pragma unroll
-    for (int j = 0; j < A; ++j) {
-        uint64_t o = 0;
-#pragma unroll
-        for (int r = 0; r < (MIRROR ? A_LOG - 1 : A_LOG); ++r)
-            if ((j >> r) & 1)
-                o |= 1ull << a.rpos[r];
-        roff[j] = o;
-    }
+    // register offsets recomputed where the loads and stores use them
+    const uint32_t rpos_w = pack_rpos(a);
+    constexpr int NB = MIRROR ? A_LOG - 1 : A_LOG;
     // both states' elements of a tile in registers (upper register half of a
     // mirror pass: the complementary tile at complemented offsets)
     auto load_tile = [&](uint64_t tl, T *p, T *l) {
-        const uint64_t b0 = unpack_runs(tl, a.hruns) + toff;
+        uint64_t roff[A];
+        reg_offsets<A, NB>(rpos_w, roff);
+        const uint64_t b0 = unpack_runs_uniform(tl, a.hruns) + toff;
 #pragma unroll
         for (int j = 0; j < AH; ++j) {
             p[j] = psi[b0 + roff[j]];
             l[j] = lam[b0 + roff[j]];
         }
         if (MIRROR) {
-            const uint64_t bm = unpack_runs(~tl & hmask, a.hruns) + a.mirror_pm - toff;
+            const uint64_t bm = unpack_runs_uniform(~tl & hmask, a.hruns) + a.mirror_pm - toff;
 #pragma unroll
             for (int j = AH; j < A; ++j) {
                 p[j] = psi[bm - roff[j - AH]];
@@ -173,7 +168,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
         }
     }
     for (uint64_t tile = first_tile; tile < end_tile; ++tile) {
-        const uint64_t base = unpack_runs(tile, a.hruns) + toff;
+        const uint64_t base = unpack_runs_uniform(tile, a.hruns) + toff;
         T wp[A], wl[A];
         if (!PREFETCH && tile != first_tile) {
             load_tile(tile, vp, vl);
@@ -304,17 +299,21 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 }
             }
         }
+        {
+            uint64_t roff[A];
+            reg_offsets<A, NB>(rpos_w, roff);
 #pragma unroll
-        for (int j = 0; j < AH; ++j) {
-            psi[base + roff[j]] = vp[j];
-            lam[base + roff[j]] = vl[j];
-        }
-        if (MIRROR) {
-            const uint64_t bm = unpack_runs(~tile & hmask, a.hruns) + a.mirror_pm - toff;
+            for (int j = 0; j < AH; ++j) {
+                psi[base + roff[j]] = vp[j];
+                lam[base + roff[j]] = vl[j];
+            }
+            if (MIRROR) {
+                const uint64_t bm = unpack_runs_uniform(~tile & hmask, a.hruns) + a.mirror_pm - toff;
 #pragma unroll
-            for (int j = AH; j < A; ++j) {
-                psi[bm - roff[j - AH]] = vp[j];
-                lam[bm - roff[j - AH]] = vl[j];
+                for (int j = AH; j < A; ++j) {
+                    psi[bm - roff[j - AH]] = vp[j];
+                    lam[bm - roff[j - AH]] = vl[j];
+                }
             }
         }
         // the next tile's LDS phases start after every thread has read its

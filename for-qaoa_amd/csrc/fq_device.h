@@ -226,6 +226,59 @@ __device__ __forceinline__ uint64_t unpack_runs(uint64_t x, const BitRuns &r)
     return o;
 }
 
+// Passes a wave-uniform word through an empty asm in a scalar register: what
+// is computed from it after this point is computed there, in scalar
+// instructions, instead of once before a tile loop.  Loop-invariant scalars
+// held across a tile loop (the 2^A_LOG 64-bit register offsets, the bit-run
+// shifts and masks of the tile index, an LDS phase's slots) do not fit the
+// scalar register file; the compiler spills them to VGPR lanes, and each use
+// then costs one v_readlane (a vector instruction) per 32 bits.
+__device__ __forceinline__ void scalar_opaque(uint32_t &x)
+{
+    asm volatile("" : "+s"(x));
+}
+
+// unpack_runs of a wave-uniform index (a tile index) with the run words
+// recomputed in scalar registers at each call (scalar_opaque).
+__device__ __forceinline__ uint64_t unpack_runs_uniform(uint64_t x, const BitRuns &r)
+{
+    uint32_t n = (uint32_t)r.n;
+    scalar_opaque(n);
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < FQ_MAX_RUNS; ++k) {
+        if ((uint32_t)k < n) {
+            uint32_t d = r.run[k];
+            scalar_opaque(d);
+            o |= ((x >> (d & 0xff)) & ((1ull << ((d >> 8) & 0xff)) - 1ull)) << (d >> 16);
+        }
+    }
+    return o;
+}
+
+// Element offsets of a thread's 2^A_LOG registers: o[j] = sum of 2^rpos[r]
+// over the set bits r < NB of j, from the packed positions rpos_w (rpos[r] in
+// byte r), recomputed in scalar registers at each call (scalar_opaque).
+template <int A, int NB>
+__device__ __forceinline__ void reg_offsets(uint32_t rpos_w, uint64_t (&o)[A])
+{
+    scalar_opaque(rpos_w);
+#pragma unroll
+    for (int j = 0; j < A; ++j) {
+        uint64_t x = 0;
+#pragma unroll
+        for (int r = 0; r < NB; ++r)
+            if ((j >> r) & 1)
+                x |= 1ull << ((rpos_w >> (8 * r)) & 63u);
+        o[j] = x;
+    }
+}
+
+__device__ __forceinline__ uint32_t pack_rpos(const TileArgs &a)
+{
+    return (uint32_t)a.rpos[0] | (uint32_t)a.rpos[1] << 8 | (uint32_t)a.rpos[2] << 16 | (uint32_t)a.rpos[3] << 24;
+}
+
 // Tile-bit mask and register gate bits of LDS phase g.  Read from packed
 // words with a shift: an indexed kernel-argument array would be loaded from
 // memory inside the group loop, and the wait for that load (the counters are

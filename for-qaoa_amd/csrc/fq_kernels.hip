@@ -86,7 +86,7 @@ __device__ __forceinline__ void phase_of(double theta, float &c, float &s)
 
 __device__ __forceinline__ uint64_t tile_base(const TileArgs &a, uint64_t tile)
 {
-    return unpack_runs(tile, a.hruns);
+    return unpack_runs_uniform(tile, a.hruns);
 }
 
 // deposit the low bits of x into the set bits of mask (q-bit masks)
@@ -225,16 +225,9 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     constexpr int AH = MIRROR ? (1 << A_LOG) / 2 : (1 << A_LOG);
 
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
-    uint64_t roff[A];
-#pragma unroll
-    for (int j = 0; j < A; ++j) {
-        uint64_t o = 0;
-#pragma unroll
-        for (int r = 0; r < (MIRROR ? A_LOG - 1 : A_LOG); ++r)
-            if ((j >> r) & 1)
-                o |= 1ull << a.rpos[r];
-        roff[j] = o;
-    }
+    // element offset of register j, recomputed where the loads and stores use it
+    const uint32_t rpos_w = pack_rpos(a);
+    constexpr int NB = MIRROR ? A_LOG - 1 : A_LOG;
     // per-thread LDS slots of every remapping phase (padded), once per launch;
     // lds_pad is additive over disjoint bit sets: pad(x | y) = pad(x) + pad(y)
     int etp[MAXG];
@@ -311,6 +304,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     }
     T v[A];
     if (!FIRST && tile < ntiles) {
+        uint64_t roff[A];
+        reg_offsets<A, NB>(rpos_w, roff);
         const T *p0 = sv + (tile_base(a, tile) + toff);
 #pragma unroll
         for (int j = 0; j < AH; ++j)
@@ -389,6 +384,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             if (ROWPF && next < ntiles)
                 fnext = ptab[next * q1 + row_lane];
             if (!FIRST && next < ntiles) {
+                uint64_t roff[A];
+                reg_offsets<A, NB>(rpos_w, roff);
                 const T *p1 = sv + (tile_base(a, next) + toff);
 #pragma unroll
                 for (int j = 0; j < AH; ++j)
@@ -405,7 +402,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         // LDS slots of LDS phase g (padded): eg for this thread, ej[j] for register j
         int ej[A];
         auto phase_slots = [&](int g) -> int {
-            const unsigned gm = group_mask(a, g);
+            uint32_t gm = group_mask(a, g);
+            scalar_opaque(gm);
             int pr[A_LOG > 0 ? A_LOG : 1];
             unsigned g2 = gm;
 #pragma unroll
@@ -543,6 +541,8 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                 ps[o] = v[j];
             }
         } else {
+            uint64_t roff[A];
+            reg_offsets<A, NB>(rpos_w, roff);
             T *const p0 = sv + (tile_base(a, tile) + toff);
 #pragma unroll
             for (int j = 0; j < AH; ++j)

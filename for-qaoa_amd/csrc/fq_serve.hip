@@ -53,16 +53,19 @@ __device__ __forceinline__ uint64_t slab_next_live(const SlotParams *__restrict_
 
 // The 2^A_LOG amplitudes of one thread in tile tl (mirror: the upper half
 // from the partner tile at complemented offsets).
-template <typename T, int A, int AH, bool MIRROR>
+template <typename T, int A_LOG, bool MIRROR>
 __device__ __forceinline__ void slab_load(T *v, const T *__restrict__ sv, const TileArgs &a, uint64_t tl,
-                                          uint64_t local_mask, uint64_t toff, const uint64_t *roff)
+                                          uint64_t local_mask, uint64_t toff, uint32_t rpos_w)
 {
-    const T *p0 = sv + (unpack_runs(tl, a.hruns) + toff);
+    constexpr int A = 1 << A_LOG, AH = MIRROR ? A / 2 : A;
+    uint64_t roff[A];
+    reg_offsets<A, MIRROR ? A_LOG - 1 : A_LOG>(rpos_w, roff);
+    const T *p0 = sv + (unpack_runs_uniform(tl, a.hruns) + toff);
 #pragma unroll
     for (int j = 0; j < AH; ++j)
         v[j] = p0[roff[j]];
     if (MIRROR) {
-        const T *pm = sv + (unpack_runs(tl ^ local_mask, a.hruns) + a.mirror_pm - toff);
+        const T *pm = sv + (unpack_runs_uniform(tl ^ local_mask, a.hruns) + a.mirror_pm - toff);
 #pragma unroll
         for (int j = AH; j < A; ++j)
             v[j] = pm[-(int64_t)roff[j - AH]];
@@ -88,16 +91,8 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CT<R>::T *__restric
     const uint64_t ntiles = 1ull << a.nh;
     const uint64_t local_mask = (1ull << slot_shift) - 1ull;
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
-    uint64_t roff[A];
-#pragma unroll
-    for (int j = 0; j < A; ++j) {
-        uint64_t o = 0;
-#pragma unroll
-        for (int r = 0; r < (MIRROR ? A_LOG - 1 : A_LOG); ++r)
-            if ((j >> r) & 1)
-                o |= 1ull << a.rpos[r];
-        roff[j] = o;
-    }
+    // register offsets recomputed where the loads and stores use them
+    const uint32_t rpos_w = pack_rpos(a);
     const int qmask = (1 << Q) - 1;
     const uint64_t q1 = (uint64_t)Q + 1;
     T *lds = reinterpret_cast<T *>(fq_smem_s);
@@ -111,16 +106,16 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CT<R>::T *__restric
     uint64_t tile = slab_next_live<MIRROR>(slots, range0, range_end, slot_shift);
     T v[A];
     if (tile < range_end && !(COST && slots[tile >> slot_shift].cost_first))
-        slab_load<T, A, AH, MIRROR>(v, sv, a, tile, local_mask, toff, roff);
+        slab_load<T, A_LOG, MIRROR>(v, sv, a, tile, local_mask, toff, rpos_w);
     while (tile < range_end) {
         const uint64_t next = slab_next_live<MIRROR>(slots, tile + 1, range_end, slot_shift);
         T w[A];
         if (next < range_end && !(COST && slots[next >> slot_shift].cost_first))
-            slab_load<T, A, AH, MIRROR>(w, sv, a, next, local_mask, toff, roff);
+            slab_load<T, A_LOG, MIRROR>(w, sv, a, next, local_mask, toff, rpos_w);
         const SlotParams &sp = slots[tile >> slot_shift];
         const uint64_t lt = tile & local_mask;
-        T *const p0 = sv + (unpack_runs(tile, a.hruns) + toff);
-        T *const pm = MIRROR ? sv + (unpack_runs(tile ^ local_mask, a.hruns) + a.mirror_pm - toff) : p0;
+        T *const p0 = sv + (unpack_runs_uniform(tile, a.hruns) + toff);
+        T *const pm = MIRROR ? sv + (unpack_runs_uniform(tile ^ local_mask, a.hruns) + a.mirror_pm - toff) : p0;
         const bool first = COST && sp.cost_first;
         if (COST) {
             const double2 *f_t = sp.ptab + lt * q1;
@@ -225,12 +220,16 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CT<R>::T *__restric
             for (int j = 0; j < A; ++j)
                 v[j] = lds[nat + lds_pad(j << nthr)];
         }
+        {
+            uint64_t roff[A];
+            reg_offsets<A, MIRROR ? A_LOG - 1 : A_LOG>(rpos_w, roff);
 #pragma unroll
-        for (int j = 0; j < AH; ++j)
-            p0[roff[j]] = v[j];
+            for (int j = 0; j < AH; ++j)
+                p0[roff[j]] = v[j];
 #pragma unroll
-        for (int j = AH; j < A; ++j)
-            pm[-(int64_t)roff[j - AH]] = v[j];
+            for (int j = AH; j < A; ++j)
+                pm[-(int64_t)roff[j - AH]] = v[j];
+        }
 #pragma unroll
         for (int j = 0; j < A; ++j)
             v[j] = w[j];
