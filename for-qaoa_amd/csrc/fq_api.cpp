@@ -471,8 +471,14 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
         a.grp_gates_w |= (uint32_t)(gg & 0xf) << (4 * a.ngroups);
         a.ngroups++;
     }
-    if (a.ngroups > 0 && g.nthr >= 6 && (a.grp_mask[a.ngroups - 1] & 0x3f) == 0)
+    a.dpos_w = 0;
+    if (a.ngroups > 0 && g.nthr >= 6 && (a.grp_mask[a.ngroups - 1] & 0x3f) == 0) {
         a.direct_store = 1;
+        // register r of the last phase is its r-th lowest tile bit
+        for (int i = 0, r = 0; i < g.Q; ++i)
+            if ((a.grp_mask[a.ngroups - 1] >> i) & 1u)
+                a.dpos_w |= (uint32_t)g.tpos[i] << (8 * r++);
+    }
     return FQ_OK;
 }
 

@@ -56,18 +56,22 @@ __device__ __forceinline__ void rx_pair_t(T &q0, T &q1, R t)
     q1 = o1;
 }
 
+// RX on the register bits r set in mask (wave-uniform).  Branch-free: an
+// ungated bit runs the butterfly with t = 0, which is the identity exactly.
+// A branch per bit would join two register assignments after each
+// butterfly, and the moves that join them cost about half as many
+// instructions as the butterfly itself.
 template <typename T, typename R, int A_LOG>
 __device__ __forceinline__ void rx_registers_t(T *v, unsigned mask, R t)
 {
     constexpr int A = 1 << A_LOG;
 #pragma unroll
     for (int r = 0; r < A_LOG; ++r) {
-        if ((mask >> r) & 1u) {
+        const R tr = ((mask >> r) & 1u) ? t : R(0);
 #pragma unroll
-            for (int j = 0; j < A; ++j)
-                if (!((j >> r) & 1))
-                    rx_pair_t<T, R>(v[j], v[j | (1 << r)], t);
-        }
+        for (int j = 0; j < A; ++j)
+            if (!((j >> r) & 1))
+                rx_pair_t<T, R>(v[j], v[j | (1 << r)], tr);
     }
 }
 

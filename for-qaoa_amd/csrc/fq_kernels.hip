@@ -241,17 +241,14 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     // Direct store: when the last LDS phase holds none of the six lane bits,
     // the tile is stored straight from that phase's registers (no write-back
     // and re-read of the natural layout).  Element e of the tile lives at
-    // tile_base + phys(e); phys is additive over disjoint bits, so the
-    // thread's part (pe_last) and the part of each of the phase's register
-    // bits (pb_last) are computed once per launch.
+    // tile_base + phys(e); phys is additive over disjoint bits: the thread's
+    // part (pe_last) is computed once per launch, the part of each register
+    // (the phase's register bits sit at the positions packed in a.dpos_w) at
+    // the stores, as reg_offsets does for the natural layout.
     // (full-width tiles only, and not in the read-write cost passes, whose
     // registers are full with the prefetch and the phase factors)
     const bool direct = GATES && MIRROR == 0 && QT > 0 && (FIRST || !COST) && a.direct_store && a.ngroups > 0;
     uint64_t pe_last = 0;
-    uint64_t pb_last[A_LOG > 0 ? A_LOG : 1];   // position of each register bit of the last phase
-#pragma unroll
-    for (int r = 0; r < A_LOG; ++r)
-        pb_last[r] = 0;
     if (direct) {
         const unsigned qmask = (1u << Q) - 1u;
         const unsigned gm = (unsigned)a.grp_mask[a.ngroups - 1];
@@ -262,15 +259,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         for (int r = 0; r < A_LOG; ++r)
             if ((et >> (nthr + r)) & 1u)
                 pe_last |= 1ull << a.rpos[r];
-        unsigned g2 = gm;
-#pragma unroll
-        for (int r = 0; r < A_LOG; ++r) {
-            const int b = __builtin_ctz(g2);
-            g2 &= g2 - 1u;
-            pb_last[r] = b < nthr ? unpack_runs(1ull << b, a.truns) : 1ull << a.rpos[b - nthr];
-        }
     }
-
     const uint64_t q1 = (uint64_t)Q + 1;
     const double2 *thr_tab = COST ? ptab + (ntiles * q1) : nullptr;
     const double2 *rr_tab = COST ? thr_tab + ((size_t)(1 + A_LOG) << nthr) : nullptr;
@@ -279,10 +268,11 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
     uint64_t tile = tile_at(a, ntiles, 0);
     // compile-time tile width: lane i of each wave holds factor i of the
     // tile's phase-table row, loaded one tile ahead with the amplitudes
-    // (not in the complex64 boundary pass: without the row and the
-    // register-register factors in VGPRs it spills 36 instead of 92 bytes per
-    // lane, and its layers measured 17% faster than in layer order)
-    constexpr bool ROWPF = COST && QT > 0 && !(PRE && sizeof(R) == 4);
+    // (not in the complex64 read-write cost passes: the boundary pass
+    // with the row and the register-register factors in VGPRs spilled 92
+    // bytes per lane, and both passes fit their 128 VGPRs without spills
+    // when they read the row with scalar loads instead)
+    constexpr bool ROWPF = COST && QT > 0 && !(sizeof(R) == 4 && !FIRST);
     const int row_lane = (t & 63) % (QT + 1);
     double2 fcur = make_double2(0.0, 0.0), fnext = make_double2(0.0, 0.0);
     if (ROWPF && tile < ntiles)
@@ -530,16 +520,12 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
         }
 
         if (direct) {
+            uint64_t doff[A];
+            reg_offsets<A, A_LOG>(a.dpos_w, doff);
             T *const ps = sv + (tile_base(a, tile) + pe_last);
 #pragma unroll
-            for (int j = 0; j < A; ++j) {
-                uint64_t o = 0;
-#pragma unroll
-                for (int r = 0; r < A_LOG; ++r)
-                    if ((j >> r) & 1)
-                        o += pb_last[r];
-                ps[o] = v[j];
-            }
+            for (int j = 0; j < A; ++j)
+                ps[doff[j]] = v[j];
         } else {
             uint64_t roff[A];
             reg_offsets<A, NB>(rpos_w, roff);
