@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     const uint64_t hmask = (1ull << a.nh) - 1ull;
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     // register offsets recomputed where the loads and stores use them
-    const uint32_t rpos_w = pack_rpos(a);
+    const uint32_t rpos_w = a.rpos_w;
     constexpr int NB = MIRROR ? A_LOG - 1 : A_LOG;
     // both states' elements of a tile in registers (upper register half of a
     // mirror pass: the complementary tile at complemented offsets)

@@ -412,8 +412,11 @@ static void geom_to_args(const Geom &g, TileArgs &a)
     a.nh = g.nh;
     make_runs(g.hpos, g.nh, a.hruns);
     make_runs(g.tpos, g.nthr, a.truns);
-    for (int r = 0; r < g.a_log; ++r)
+    a.rpos_w = 0;
+    for (int r = 0; r < g.a_log; ++r) {
         a.rpos[r] = g.tpos[g.nthr + r];
+        a.rpos_w |= (uint32_t)g.tpos[g.nthr + r] << (6 * r);
+    }
 }
 
 static std::string geom_key(const Geom &g)
@@ -468,7 +471,7 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
         a.grp_gates[a.ngroups] = gg;
         if (a.ngroups < 4)
             a.grp_mask_w |= (uint64_t)gm << (16 * a.ngroups);
-        a.grp_gates_w |= (uint32_t)(gg & 0xf) << (4 * a.ngroups);
+        a.grp_gates_w |= (uint32_t)(gg & 0x1f) << (5 * a.ngroups);
         a.ngroups++;
     }
     a.dpos_w = 0;
@@ -477,7 +480,7 @@ static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
         // register r of the last phase is its r-th lowest tile bit
         for (int i = 0, r = 0; i < g.Q; ++i)
             if ((a.grp_mask[a.ngroups - 1] >> i) & 1u)
-                a.dpos_w |= (uint32_t)g.tpos[i] << (8 * r++);
+                a.dpos_w |= (uint32_t)g.tpos[i] << (6 * r++);
     }
     return FQ_OK;
 }
@@ -557,7 +560,7 @@ static void build_plan(const fq_graph *G, const Geom &g, PlanBlob &pb)
         thr[t] = base;
     }
     PlanDev &dev = pb.dev;
-    for (int j = 0; j < 16; ++j)
+    for (int j = 0; j < FQ_MAX_REGS; ++j)
         dev.rr[j] = 0;
     for (int j = 0; j < A; ++j) {
         double v = 0;
@@ -740,10 +743,49 @@ static int check_state(const fq_state *st)
     return FQ_OK;
 }
 
-static int tile_a_log(int dtype, int Q)
+// FQ_WIDE_REGS=0 (environment, read per call): the contiguous 14-qubit
+// complex64 tile passes always run 1024 threads x 16 amplitudes (tile_a_log).
+static bool wide_regs()
+{
+    const char *e = getenv("FQ_WIDE_REGS");
+    return e ? atoi(e) != 0 : true;
+}
+
+// Register bits per thread of a packed slab pass.
+static int slab_a_log(int dtype, int Q)
 {
     const int want = dtype == FQ_C128 ? 3 : 4;
     return std::max(1, std::min(want, Q));
+}
+
+// Register bits per thread of a tile pass.  The contiguous full-width
+// complex64 tile (qubits 0..13, 128 KiB of LDS) can run 512 threads x 32
+// amplitudes instead of 1024 x 16: five register bits per LDS phase, and the
+// 256 VGPRs of a 512-thread group hold the 32 amplitudes plus the next tile's
+// prefetch.  It is used where it needs fewer LDS phases than four bits would
+// (N=34: RX on qubits 4..13 in one phase instead of two, one of them half
+// filler), which measured on MI355X: boundary pass 61.6 -> 55.6 ms, plain
+// 48.4 -> 47.1 ms, layer-0 pass 37 -> 34 ms, 8.27 -> 8.39 layers/s.  With the
+// same number of phases the wider tile only adds filler butterflies and
+// halves the waves per CU (N=30: 130.2 vs 127.9 layers/s), and the strided
+// tiles measured slower with it (N=34 plain strided pass 55.4 -> 60.5 ms).
+static int tile_a_log(int dtype, const std::vector<int> &tile, const std::vector<int> &gates)
+{
+    const int Q = (int)tile.size();
+    bool low = true;
+    for (int i = 0; i < Q; ++i)
+        low = low && tile[i] == i;
+    if (dtype == FQ_C64 && Q == 14 && low && wide_regs()) {
+        // gated thread bits (tile position = qubit here) and the LDS phases they need
+        int g4 = 0, g5 = 0;
+        for (int q : gates) {
+            g4 += q < 10;
+            g5 += q < 9;
+        }
+        if ((g5 + 4) / 5 < (g4 + 3) / 4)
+            return 5;
+    }
+    return slab_a_log(dtype, Q);
 }
 
 extern "C" int fq_state_create(int N, int D, int B, int C, int dtype, fq_state **out)
@@ -995,7 +1037,7 @@ static void set_cost(TileArgs &a, double gamma, double init_amp)
     a.gamma = gamma;
     a.init_amp = init_amp;
     a.has_rr = 0;
-    for (int j = 0; j < 16; ++j)
+    for (int j = 0; j < FQ_MAX_REGS; ++j)
         if (a.plan.rr[j] != 0)
             a.has_rr = 1;
 }
@@ -1054,7 +1096,7 @@ static int pass_args(int N, int dtype, const std::vector<int> &tile, const std::
                      int z2, TileArgs &a, Geom &geom, int a_log_override = 0)
 {
     const int Q = (int)tile.size();
-    const int a_log = a_log_override > 0 ? a_log_override : tile_a_log(dtype, Q);
+    const int a_log = a_log_override > 0 ? a_log_override : tile_a_log(dtype, tile, gates);
     geom = make_geom(N, tile, a_log);
     if (geom_runs(geom) > FQ_MAX_RUNS)
         return set_err(FQ_ERR_ARG, "internal: tile geometry needs more than %d bit runs", FQ_MAX_RUNS);
@@ -1121,6 +1163,8 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     absorb(beta, &a.rx_t);
     if (pre_beta)
         absorb(*pre_beta, &a.pre_t);
+    a.rx_tf = (float)a.rx_t;   // exact for complex64 (rx_tform rounds t to FP32)
+    a.pre_tf = (float)a.pre_t;
     if (cost || !defer) {
         a.scale_re = sr;
         a.scale_im = si;
@@ -1250,7 +1294,7 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype, int bound
             std::vector<int> &nw = news[k2];
             std::sort(nw.begin(), nw.end());
             const int c = Qmax - want[k2];
-            const int a_log = tile_a_log(dtype, Qmax);
+            const int a_log = slab_a_log(dtype, Qmax);
             // lane order: the 64 lanes of a wave are the tile's first six
             // positions; past the carriers they take the pass's lowest new
             // qubits whose byte-address bit is outside [19, hi] (one load
@@ -1280,7 +1324,7 @@ static std::vector<FusedPass> cover_passes(int N, int Qmax, int dtype, int bound
                 if (std::find(lane_q.begin(), lane_q.end(), q) == lane_q.end())
                     p.tile.push_back(q);
             p.gates = nw;
-            fits = fits && geom_runs(make_geom(N, p.tile, tile_a_log(dtype, Qmax))) <= FQ_MAX_RUNS;
+            fits = fits && geom_runs(make_geom(N, p.tile, slab_a_log(dtype, Qmax))) <= FQ_MAX_RUNS;
             more.push_back(p);
         }
         if (fits || blk >= rest) {
@@ -1355,7 +1399,7 @@ static int rx_targets_once(fq_state *st, std::vector<int> targets, double beta)
                 if (std::find(chunk.begin(), chunk.end(), p) == chunk.end())
                     tile.push_back(p);
             std::sort(tile.begin(), tile.end());
-            const int a_log = tile_a_log(st->dtype, (int)tile.size());
+            const int a_log = tile_a_log(st->dtype, tile, chunk);
             if (chunk.size() == 1 || geom_runs(make_geom(N, tile, a_log)) <= FQ_MAX_RUNS)
                 break;
             chunk.pop_back();  // scattered targets: fewer per pass
@@ -2256,7 +2300,7 @@ static int slab_qe(const fq_slab *sb) { return std::min(12, sb->z2 ? sb->N - 1 :
 static TileArgs slab_cost_args(const fq_slab *sb, Geom *gout)
 {
     const int Q1 = (int)sb->passes[0].tile.size();
-    Geom g1 = make_geom(sb->N, sb->passes[0].tile, tile_a_log(sb->dtype, Q1));
+    Geom g1 = make_geom(sb->N, sb->passes[0].tile, slab_a_log(sb->dtype, Q1));
     TileArgs a1;
     geom_to_args(g1, a1);
     if (sb->z2)
@@ -2278,7 +2322,7 @@ static int slab_build_args(fq_slab *sb)
         const FusedPass &ps = sb->passes[i];
         const int Q = (int)ps.tile.size();
         SlabPass sp;
-        sp.a_log = tile_a_log(sb->dtype, Q);
+        sp.a_log = slab_a_log(sb->dtype, Q);
         sp.mirror = sb->z2 && i + 1 == sb->passes.size();
         std::vector<int> tile = ps.tile;
         if (sp.mirror)
@@ -2363,7 +2407,7 @@ static int slab_slot_plans(fq_slab *sb, SlotHost &h)
     if (int rc = get_plan(h.g, sb->exp_geom, &h.eplan, sb->stream))
         return rc;
     h.has_rr = 0;
-    for (int j = 0; j < 16; ++j)
+    for (int j = 0; j < FQ_MAX_REGS; ++j)
         if (h.plan.rr[j] != 0)
             h.has_rr = 1;
     h.gversion = h.g->version;

@@ -160,7 +160,7 @@ __device__ __forceinline__ bool phase_coef(const TileArgs &a, const PlanDev &p, 
         coef = p.thr[gid - ntile_e];
         return true;
     }
-    if (gid < ntile_e + nthr_e + 16) {
+    if (gid < ntile_e + nthr_e + FQ_MAX_REGS) {
         coef = p.rr[gid - ntile_e - nthr_e];
         return true;
     }
@@ -260,27 +260,22 @@ __device__ __forceinline__ uint64_t unpack_runs_uniform(uint64_t x, const BitRun
     return o;
 }
 
-// Element offsets of a thread's 2^A_LOG registers: o[j] = sum of 2^rpos[r]
-// over the set bits r < NB of j, from the packed positions rpos_w (rpos[r] in
-// byte r), recomputed in scalar registers at each call (scalar_opaque).
+// Element offsets of a thread's 2^A_LOG registers: o[j] = sum of 2^pos[r]
+// over the set bits r < NB of j, from the packed positions pos_w (pos[r] in
+// bits 6r..6r+5), recomputed in scalar registers at each call (scalar_opaque).
 template <int A, int NB>
-__device__ __forceinline__ void reg_offsets(uint32_t rpos_w, uint64_t (&o)[A])
+__device__ __forceinline__ void reg_offsets(uint32_t pos_w, uint64_t (&o)[A])
 {
-    scalar_opaque(rpos_w);
+    scalar_opaque(pos_w);
 #pragma unroll
     for (int j = 0; j < A; ++j) {
         uint64_t x = 0;
 #pragma unroll
         for (int r = 0; r < NB; ++r)
             if ((j >> r) & 1)
-                x |= 1ull << ((rpos_w >> (8 * r)) & 63u);
+                x |= 1ull << ((pos_w >> (6 * r)) & 63u);
         o[j] = x;
     }
-}
-
-__device__ __forceinline__ uint32_t pack_rpos(const TileArgs &a)
-{
-    return (uint32_t)a.rpos[0] | (uint32_t)a.rpos[1] << 8 | (uint32_t)a.rpos[2] << 16 | (uint32_t)a.rpos[3] << 24;
 }
 
 // Tile-bit mask and register gate bits of LDS phase g.  Read from packed
@@ -296,7 +291,7 @@ __device__ __forceinline__ unsigned group_mask(const TileArgs &a, int g)
 
 __device__ __forceinline__ unsigned group_gates(const TileArgs &a, int g)
 {
-    return (a.grp_gates_w >> (4 * g)) & 0xfu;
+    return (a.grp_gates_w >> (5 * g)) & 0x1fu;
 }
 
 }  // namespace fq

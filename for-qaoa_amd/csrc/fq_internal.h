@@ -12,6 +12,8 @@
 #define FQ_MAX_GROUPS 5    // LDS re-mapping phases per tile pass
 #define FQ_MAX_RUNS 8      // bit runs of a scattered index
 #define FQ_TOPK_MAX 16
+#define FQ_MAX_REG_LOG 5   // register bits per thread of a tile pass
+#define FQ_MAX_REGS (1 << FQ_MAX_REG_LOG)
 
 namespace fq {
 
@@ -40,7 +42,7 @@ struct PlanDev {
     int n_hh;
     int pad_;
     double wtot;           // sum of all weights
-    double rr[16];         // register-register cut contributions
+    double rr[FQ_MAX_REGS];  // register-register cut contributions
 };
 
 // Index bits [src, src+len) go to physical positions [dst, dst+len); one run
@@ -56,7 +58,8 @@ struct TileArgs {
     int N, Q, nthr, nh;
     BitRuns hruns;           // tile index -> high positions
     BitRuns truns;           // thread index -> positions of the thread bits
-    uint8_t rpos[4];         // positions of the register bits
+    uint8_t rpos[8];         // positions of the register bits
+    uint32_t rpos_w;         // the same packed, 6 bits each (rpos[r] at bit 6r)
     uint64_t ntiles_run;     // tiles processed (0: all 2^nh)
     uint64_t mirror_pm;      // mirror pass: sum of 2^p over the real tile positions
     uint32_t order;          // tile order of the persistent loop (tile_at)
@@ -65,14 +68,15 @@ struct TileArgs {
     uint16_t grp_mask[FQ_MAX_GROUPS];  // tile-bit mask of each LDS phase (A_LOG bits)
     uint8_t grp_gates[FQ_MAX_GROUPS];  // register bits of that phase that receive RX
     uint64_t grp_mask_w;     // grp_mask[0..3], 16 bits each (read with a shift, not an index)
-    uint32_t grp_gates_w;    // grp_gates[0..4], 4 bits each
+    uint32_t grp_gates_w;    // grp_gates[0..4], 5 bits each
     uint32_t direct_store;   // last LDS phase holds no lane bit: store from its layout
-    uint32_t dpos_w;         // direct_store: position of the last phase's register bit r (byte r)
+    uint32_t dpos_w;         // direct_store: position of the last phase's register bit r (bit 6r)
     double rx_c, rx_s;       // cos(beta/2), sin(beta/2) (host-computed, src/rx.cu:121)
     double pre_c, pre_s;     // layer-boundary pass: RX of the previous layer, applied first
     // k_tile applies RX up to a factor as the butterfly [[1, -it], [-it, 1]]
     // (rx_tform), one FMA per output component
     double rx_t, pre_t;
+    float rx_tf, pre_tf;     // the same in FP32 (scalar operands of the complex64 butterflies)
     double scale_re, scale_im;  // factor of the pass: folded into the cost phase,
     int apply_scale;            // or (apply_scale) multiplied in before the stores
     double gamma, init_amp;  // cost phase angle, launch-control amplitude

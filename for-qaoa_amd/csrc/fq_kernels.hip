@@ -205,7 +205,7 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
 // pass folds into its FP64 phase factor and a pass without the cost phase
 // multiplies in before its stores when a.apply_scale is set.
 template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false>
-__global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a,
+__global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a,
                                                             const double2 *__restrict__ ptab)
 {
     typedef typename CT<R>::T T;
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
 
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     // element offset of register j, recomputed where the loads and stores use it
-    const uint32_t rpos_w = pack_rpos(a);
+    const uint32_t rpos_w = a.rpos_w;
     constexpr int NB = MIRROR ? A_LOG - 1 : A_LOG;
     // per-thread LDS slots of every remapping phase (padded), once per launch;
     // lds_pad is additive over disjoint bit sets: pad(x | y) = pad(x) + pad(y)
@@ -321,13 +321,19 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             auto row = [&](int i) -> double2 { return ROWPF ? lane_bcast(fcur, i) : f_t[i]; };
             double x = f_thr.x, y = f_thr.y;
             zmul(x, y, row(Q));
+            // thread bits from an opaque copy of t: the per-bit lane masks
+            // of the selects below are recomputed per tile (two vector
+            // instructions each) instead of held in scalar registers across
+            // the tile loop, where they spill
+            int tb = t;
+            lds_opaque(tb);
 #pragma unroll
             for (int i = 0; i < (QT > 0 ? QT - A_LOG : FQ_MAXQ); ++i) {
                 if (QT > 0 || i < nthr) {
                     const double2 f = row(i);
                     const double nx = x * f.x - y * f.y;
                     const double ny = x * f.y + y * f.x;
-                    const bool on = (t >> i) & 1;
+                    const bool on = (tb >> i) & 1;
                     x = on ? nx : x;
                     y = on ? ny : y;
                 }
@@ -450,7 +456,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
             // then the other phases and the natural register bits: two LDS
             // round trips fewer than finishing RX(pre) in the natural layout
             // and starting again.
-            const R pt = (R)a.pre_t, rt = (R)a.rx_t;
+            const R pt = a.pre_tf, rt = a.rx_tf;
             const int G = a.ngroups;
             rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, pt);
             int eg = 0;
@@ -488,7 +494,7 @@ __global__ void __launch_bounds__(GATES ? 1024 : 256) k_tile(typename CT<R>::T *
                 apply_cost();
             prefetch();
             if (GATES) {
-                const R rt = (R)a.rx_t;
+                const R rt = sizeof(R) == 4 ? (R)a.rx_tf : (R)a.rx_t;
                 rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, rt);
                 if (a.ngroups > 0) {
                     natural_write();
@@ -1155,6 +1161,8 @@ hipError_t launch_tile_mirror(int dtype, int a_log, bool expand, void *sv, const
             default: return hipErrorInvalidValue;
         }
     }
+    if (a_log == 5 && a.Q == 14)
+        return launch_mirror_t<float, 5, 14>(expand, sv, a, stream);
     if (a_log == 4 && a.Q == 14)
         return launch_mirror_t<float, 4, 14>(expand, sv, a, stream);
     switch (a_log) {
@@ -1204,7 +1212,7 @@ static hipError_t launch_tile_r(int a_log, bool cost, bool first, bool gates, vo
     }
     // compile-time tile widths for the full-width tiles of the fused passes;
     // anything else takes the generic instance.  Register bits per thread:
-    // up to 3 for complex128, 4 for complex64.
+    // up to 3 for complex128, 4 for complex64 (5 in the 14-qubit tiles).
     if constexpr (sizeof(R) == 8) {
         if (a_log == 3 && a.Q == 13)
             return launch_gated<R, 3, 13>(cost, first, sv, a, ptab, stream);
@@ -1217,6 +1225,8 @@ static hipError_t launch_tile_r(int a_log, bool cost, bool first, bool gates, vo
             default: return hipErrorInvalidValue;
         }
     } else {
+        if (a_log == 5 && a.Q == 14)
+            return launch_gated<R, 5, 14>(cost, first, sv, a, ptab, stream);
         if (a_log == 4 && a.Q == 14)
             return launch_gated<R, 4, 14>(cost, first, sv, a, ptab, stream);
         if (a_log == 4 && a.Q == 13)
@@ -1250,6 +1260,8 @@ static hipError_t launch_pre_r(int a_log, void *sv, const TileArgs &a, const dou
         if (a_log == 3 && a.Q == 12)
             return launch_tile_t<R, 3, 12, true, false, true, true>(sv, a, ptab, stream);
     } else {
+        if (a_log == 5 && a.Q == 14)
+            return launch_tile_t<R, 5, 14, true, false, true, true>(sv, a, ptab, stream);
         if (a_log == 4 && a.Q == 14)
             return launch_tile_t<R, 4, 14, true, false, true, true>(sv, a, ptab, stream);
         if (a_log == 4 && a.Q == 13)
@@ -1320,7 +1332,7 @@ hipError_t launch_stream(void *p, size_t bytes, int mode, float *sink, hipStream
 
 size_t phase_table_entries(const TileArgs &a, int a_log)
 {
-    return (size_t)run_tiles(a) * (size_t)(a.Q + 1) + ((size_t)(1 + a_log) << a.nthr) + 16;
+    return (size_t)run_tiles(a) * (size_t)(a.Q + 1) + ((size_t)(1 + a_log) << a.nthr) + FQ_MAX_REGS;
 }
 
 hipError_t launch_phase_table(const TileArgs &a, int a_log, double2 *tab, double *cut,

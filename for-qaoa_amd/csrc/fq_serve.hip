@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(1024) k_slab_pass(typename CT<R>::T *__restric
     const uint64_t local_mask = (1ull << slot_shift) - 1ull;
     const uint64_t toff = unpack_runs((uint64_t)t, a.truns);
     // register offsets recomputed where the loads and stores use them
-    const uint32_t rpos_w = pack_rpos(a);
+    const uint32_t rpos_w = a.rpos_w;
     const int qmask = (1 << Q) - 1;
     const uint64_t q1 = (uint64_t)Q + 1;
     T *lds = reinterpret_cast<T *>(fq_smem_s);

@@ -10,7 +10,9 @@ different tile layout or launch order, so each is a separate product path:
                  2 is the default);
 * FQ_PASS_BLOCK  block size in which the high qubits are dealt to the strided
                  passes (0: one contiguous block per pass; 2 is the default);
-* FQ_CARRIER_GATES=0  the carried low qubits' RX stays in the cost pass.
+* FQ_CARRIER_GATES=0  the carried low qubits' RX stays in the cost pass;
+* FQ_WIDE_REGS=0  the contiguous 14-qubit complex64 tiles run 1024 threads
+                 x 16 amplitudes instead of 512 x 32.
 
 States in logical order within 1e-12 (complex128) / 1e-5 (complex64) of the
 oracle's final state, at sizes where the strided passes exist (N > tile
@@ -28,7 +30,7 @@ from forqaoa import graphs as G
 pytestmark = pytest.mark.gpu
 
 TOL = {"c128": 1e-12, "c64": 1e-5}
-KNOBS = ["FQ_TILE_Q", "FQ_TILE_ORDER", "FQ_PASS_BLOCK", "FQ_CARRIER_GATES", "FQ_LAYER_PIPE"]
+KNOBS = ["FQ_TILE_Q", "FQ_TILE_ORDER", "FQ_PASS_BLOCK", "FQ_CARRIER_GATES", "FQ_LAYER_PIPE", "FQ_WIDE_REGS"]
 
 CASES = [
     ({"FQ_TILE_Q": "10"}, 22, "c128"),
@@ -44,6 +46,13 @@ CASES = [
     ({"FQ_CARRIER_GATES": "0"}, 23, "c128"),
     ({"FQ_CARRIER_GATES": "0"}, 24, "c64"),
     ({"FQ_CARRIER_GATES": "0", "FQ_LAYER_PIPE": "0"}, 22, "c64"),
+    # 512 x 32 is the default where it saves an LDS phase of the contiguous
+    # pass: at N=23 and 24 (and 32..34) its RX set (qubits 5..13 or 4..13)
+    # takes one phase of five bits instead of two of four
+    ({"FQ_WIDE_REGS": "1"}, 24, "c64"),
+    ({"FQ_WIDE_REGS": "1", "FQ_LAYER_PIPE": "0"}, 23, "c64"),
+    ({"FQ_WIDE_REGS": "0"}, 24, "c64"),
+    ({"FQ_WIDE_REGS": "0", "FQ_LAYER_PIPE": "0"}, 23, "c64"),
 ]
 
 
