@@ -321,12 +321,14 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_til
             auto row = [&](int i) -> double2 { return ROWPF ? lane_bcast(fcur, i) : f_t[i]; };
             double x = f_thr.x, y = f_thr.y;
             zmul(x, y, row(Q));
-            // thread bits from an opaque copy of t: the per-bit lane masks
-            // of the selects below are recomputed per tile (two vector
-            // instructions each) instead of held in scalar registers across
-            // the tile loop, where they spill
+            // complex64: thread bits from an opaque copy of t, so the
+            // per-bit lane masks of the selects below are recomputed per tile
+            // (two vector instructions each) instead of held in scalar
+            // registers across the tile loop, where they spill (the
+            // complex128 passes hold them without spilling)
             int tb = t;
-            lds_opaque(tb);
+            if (sizeof(R) == 4)
+                lds_opaque(tb);
 #pragma unroll
             for (int i = 0; i < (QT > 0 ? QT - A_LOG : FQ_MAXQ); ++i) {
                 if (QT > 0 || i < nthr) {
