@@ -8,6 +8,10 @@ The oracle cannot run at these sizes; size-independent pins instead:
 * N=32, P=10, G(32, 0.3) integer weights: the complex64 run against the
   complex128 run of the same instance (the error of the complex64 path is
   printed and bounded);
+* N=34, P=10, the headline complex64 instance, against complex128: a full
+  complex128 state of 34 qubits (275 GB) does not fit beside the rest, but
+  the half state z_33 = 0 does -- a one-slot complex128 half-state slab
+  (2^33 x 16 B = 137 GB), whose <C> is the state's (psi(b) = psi(~b));
 * norm preservation.
 """
 import numpy as np
@@ -17,6 +21,8 @@ import golden_util as GU
 import refmath as R
 from forqaoa import Graph, State
 from forqaoa import graphs as G
+from forqaoa import _lib
+from forqaoa.serve import Slab
 
 pytestmark = pytest.mark.gpu
 
@@ -95,3 +101,35 @@ def test_n32_c64_error_vs_c128():
     assert rel < 1e-5
     assert amp_err < 1e-3
     assert abs(n128 - 1) < 1e-11
+
+
+def test_n34_c64_p10_vs_c128_half_state():
+    w = G.workload("n34_gnp03_int")
+    N = 34
+    _lib.check(_lib.lib().fq_set_memory_budget(0))   # no budget (another test may have left one)
+    g = Graph(N, w["edges"])
+    try:
+        st = State(N, C=14, dtype="c64")
+        try:
+            st.run(g, w["gammas"], w["betas"], "fused")
+            e64, n64 = st.expect_cut(g)
+        finally:
+            st.destroy()
+        print(f"N=34 c64 P=10 fused: <C> {e64:.12g}, norm2-1 {n64 - 1:.2e}", flush=True)
+        sb = Slab(N, 1, "c128", "z2")
+        try:
+            sb.assign(0, g, type("R", (), {"gammas": w["gammas"], "betas": w["betas"]})())
+            out = []
+            while not out:
+                sb.step()
+                out = sb.collect()
+            e128, n128 = out[0].expect, out[0].norm2
+        finally:
+            sb.destroy()
+    finally:
+        g.destroy()
+    rel = abs(e64 - e128) / abs(e128)
+    print(f"N=34 P=10 G(34,0.3) int: complex128 half-state <C> {e128:.12g} (norm2-1 {n128 - 1:.2e}); "
+          f"complex64 rel err {rel:.2e}", flush=True)
+    assert abs(n128 - 1) < 1e-11
+    assert rel < 1e-5
