@@ -2234,6 +2234,15 @@ struct SlabPass {
     bool mirror = false;
 };
 
+// Layers per step of a new slab of large slots (fq_slab_set_layers_per_step):
+// FQ_SLAB_LAYERS (environment, read per slab creation), default 1.
+static int slab_layers_default()
+{
+    const char *e = getenv("FQ_SLAB_LAYERS");
+    const int v = e ? atoi(e) : 1;
+    return v >= 1 ? v : 1;
+}
+
 struct fq_slab {
     int N = 0, K = 0, s = 0, dtype = 0;
     bool z2 = false;  // slots hold the half state z_{N-1} = 0
@@ -2263,6 +2272,7 @@ struct fq_slab {
     bool per_slot = false;
     LayerPlan slot_plan;
     std::vector<fq_state *> views;
+    int layers_per_step = 1;   // per_slot slabs: layers a step runs per slot
 };
 
 static void slab_free(fq_slab *sb)
@@ -2492,6 +2502,7 @@ extern "C" int fq_slab_create_sched(int N, int nslots, int dtype, int schedule, 
         return set_err(FQ_ERR_OOM, "slab scratch allocation failed");
     }
     own_stream(sb->stream, true);
+    sb->layers_per_step = slab_layers_default();
     if (sb->per_slot)
         if (int rc = slab_make_views(sb)) {
             slab_free(sb);
@@ -2605,6 +2616,8 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
             }
         }
     }
+    // layers of this step per slot (per_slot slabs may run several)
+    std::vector<int> nl(K, 0);
     int nact = 0, nfin = 0;
     for (int k = 0; k < K; ++k) {
         SlotHost &h = sb->slots[k];
@@ -2612,6 +2625,7 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         memset(&sp, 0, sizeof(sp));
         if (h.phase != 1)
             continue;
+        nl[k] = sb->per_slot ? std::min(sb->layers_per_step, h.P - h.layer) : 1;
         const int L = h.layer;
         rx_coeffs(h.betas[L], sb->dtype, &sp.rx_c, &sp.rx_s);
         sp.gamma = h.gammas[L];
@@ -2620,7 +2634,7 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         sp.init_amp = sb->init_amp * (sb->per_slot ? 1.0 : h.amp_scale);
         sp.active = 1;
         sp.cost_first = L == 0;
-        sp.want_expect = L + 1 == h.P;
+        sp.want_expect = L + nl[k] == h.P;
         sp.ptab = sb->d_tabs + sb->tab_stride * (size_t)k;
         sp.plan = h.plan;
         sp.eplan = h.eplan;
@@ -2635,22 +2649,27 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
     FQ_HIP(hipMemcpyAsync(sb->d_params, sb->h_params, sizeof(SlotParams) * K, hipMemcpyHostToDevice,
                           sb->stream));
     if (sb->per_slot) {
-        // each slot one layer of fq_run's pipelined schedule on its own amplitudes
-        for (int k = 0; k < K; ++k) {
-            const SlotParams &sp = sb->h_params[k];
-            if (!sp.active)
-                continue;
-            SlotHost &h = sb->slots[k];
-            fq_state *v = sb->views[k];
-            v->init_amp = sp.init_amp;
-            if (h.layer == 0) {
-                v->pending_re = 1.0;
-                v->pending_im = 0.0;
+        // each slot nl[k] layers of fq_run's pipelined schedule on its own
+        // amplitudes (the slots' layers interleaved on the slab's stream)
+        const int most = *std::max_element(nl.begin(), nl.end());
+        for (int rep = 0; rep < most; ++rep) {
+            for (int k = 0; k < K; ++k) {
+                const SlotParams &sp = sb->h_params[k];
+                if (!sp.active || rep >= nl[k])
+                    continue;
+                SlotHost &h = sb->slots[k];
+                fq_state *v = sb->views[k];
+                const int L = h.layer + rep;
+                v->init_amp = sp.init_amp;
+                if (L == 0) {
+                    v->pending_re = 1.0;
+                    v->pending_im = 0.0;
+                }
+                v->in_run = true;   // a slot runs fq_run's layers on a QAOA state
+                if (int rc = run_layer(v, h.g, sb->slot_plan, L, h.P, h.gammas.data(), h.betas.data(), sb->z2,
+                                       false))
+                    return rc;
             }
-            v->in_run = true;   // a slot runs fq_run's layers on a QAOA state
-            if (int rc = run_layer(v, h.g, sb->slot_plan, h.layer, h.P, h.gammas.data(), h.betas.data(), sb->z2,
-                                   false))
-                return rc;
         }
     } else {
         FQ_HIP(launch_slab_phase_table(sb->cost_args, sb->cost_geom.a_log, sb->d_params, K, sb->d_tabs,
@@ -2672,10 +2691,20 @@ extern "C" int fq_slab_step(fq_slab *sb, int *nfinishing)
         SlotHost &h = sb->slots[k];
         if (h.phase != 1)
             continue;
-        h.layer++;
+        h.layer += nl[k];
         if (h.layer == h.P)
             h.phase = 2;
     }
+    return FQ_OK;
+}
+
+extern "C" int fq_slab_set_layers_per_step(fq_slab *sb, int layers)
+{
+    if (!sb)
+        return set_err(FQ_ERR_ARG, "null slab");
+    if (layers < 1)
+        return set_err(FQ_ERR_ARG, "layers per step must be >= 1");
+    sb->layers_per_step = layers;
     return FQ_OK;
 }
 

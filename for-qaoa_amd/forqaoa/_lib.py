@@ -94,6 +94,7 @@ SIGNATURES = {
     "fq_slab_info": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "fq_slab_assign": (_i, [_vp, _i, _vp, _i, _dp, _dp]),
     "fq_slab_step": (_i, [_vp, _ip]),
+    "fq_slab_set_layers_per_step": (_i, [_vp, _i]),
     "fq_slab_poll": (_i, [_vp, _ip]),
     "fq_slab_collect": (_i, [_vp, _ip, _dp, _dp, _ip]),
     "fq_slab_copy_out": (_i, [_vp, _i, _vp, _sz]),

@@ -131,6 +131,11 @@ class Slab:
                                    b.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         self.slot_req[slot] = req
 
+    def set_layers_per_step(self, layers):
+        """Layers a step runs per slot (slabs of large slots; see
+        fq_slab_set_layers_per_step)."""
+        check(lib().fq_slab_set_layers_per_step(self.h, int(layers)))
+
     def step(self):
         n = ctypes.c_int()
         check(lib().fq_slab_step(self.h, ctypes.byref(n)))

@@ -212,6 +212,12 @@ int fq_slab_assign(fq_slab *sb, int slot, fq_graph *g, int P, const double *gamm
 /* Enqueue one layer for every active slot (waits for the previous step first);
  * nfinishing (may be NULL) receives the number of slots finishing with it. */
 int fq_slab_step(fq_slab *sb, int *nfinishing);
+/* Layers a step runs per slot in a slab of large slots (2^10 or more
+ * first-pass tiles per slot, whose slots run fq_run's tile passes one after
+ * the other): up to `layers` (>= 1, default 1), fewer for a slot that has
+ * fewer left.  Slabs of small slots (one packed launch per pass) keep one
+ * layer per step. */
+int fq_slab_set_layers_per_step(fq_slab *sb, int layers);
 /* busy = 1 while the last step is still running (non-blocking). */
 int fq_slab_poll(fq_slab *sb, int *busy);
 /* Finished slots: indices, <C>, sum |psi|^2; frees them.  Waits. */

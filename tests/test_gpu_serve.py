@@ -251,3 +251,44 @@ def solo_expect(N, edges, g, b, dtype):
     finally:
         st.destroy()
         gr.destroy()
+
+
+@pytest.mark.parametrize("layers", [2, 3, 10])
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_large_slots_several_layers_per_step(dtype, layers):
+    # slots of 2^10 or more first-pass tiles run fq_run's passes per slot; a
+    # step may run several layers of each (fewer for a slot with fewer left)
+    N = 25
+    sb = Slab(N, 4, dtype, "z2")
+    sb.set_layers_per_step(layers)
+    jobs = {}
+    try:
+        for k in range(4):
+            edges = G.gnp(N, 0.25 + 0.15 * k, 900 + k, "uniform")
+            P = 2 + 2 * k
+            g, b = G.angles(P, 910 + k)
+            gr = Graph(N, edges)
+            jobs[k] = (gr, edges, g, b)
+            sb.assign(k, gr, type("R", (), {"gammas": g, "betas": b})())
+        got = {}
+        steps = 0
+        while len(got) < 4:
+            sb.step()
+            steps += 1
+            for r in sb.collect():
+                got[len(got)] = r
+        assert steps == -(-8 // layers)   # the longest request has P = 8
+        by_p = {len(r.gammas): r for r in got.values()}
+        worst = 0.0
+        for k, (gr, edges, g, b) in jobs.items():
+            e_solo = solo_expect(N, edges, g, b, dtype)
+            r = by_p[len(g)]
+            worst = max(worst, abs(r.expect - e_solo) / abs(e_solo))
+            assert r.norm2 == pytest.approx(1.0, abs=1e-5 if dtype == "c64" else 1e-11)
+        print(f"N=25 {dtype} {layers} layers per step: max relative |<C>_slab - <C>_alone| = {worst:.2e}",
+              flush=True)
+        assert worst < TOL[dtype]
+    finally:
+        sb.destroy()
+        for gr, *_ in jobs.values():
+            gr.destroy()
