@@ -197,16 +197,16 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
 // QT > 0 fixes the tile width at compile time (the fused passes' full-width
 // tiles); QT == 0 reads it from the arguments.
 // PRE: the pass that ends one layer and opens the next applies RX of the
-// previous layer (pre_t, pre_form) on the same gate positions before the cost
-// phase (COST, not FIRST, MIRROR 0 only).
+// previous layer (pre_t) on the same gate positions before the cost phase
+// (COST, not FIRST, MIRROR 0 only).
 // RX is applied as the one-FMA butterfly of rx_tform, i.e. up to a factor per
 // gate; the host passes the product of those factors (and of any earlier
 // passes that left theirs pending) as a.scale_re + i a.scale_im, which a cost
 // pass folds into its FP64 phase factor and a pass without the cost phase
 // multiplies in before its stores when a.apply_scale is set.
 template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false>
-__global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a,
-                                                            const double2 *__restrict__ ptab)
+__global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
+    k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a, const double2 *__restrict__ ptab)
 {
     typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
@@ -273,6 +273,18 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_til
     // bytes per lane, and both passes fit their 128 VGPRs without spills
     // when they read the row with scalar loads instead)
     constexpr bool ROWPF = COST && QT > 0 && !(sizeof(R) == 4 && !FIRST);
+    // When the next tile's loads are issued (measured on MI355X; a tile's
+    // loads and the previous tile's stores issued back to back left the
+    // memory pipe idle through the compute between them):
+    // * the complex64 boundary pass at 1024 threads issues half of them
+    //   before its cost phase and half after it (N=34: strided boundary
+    //   pass 72.5 -> 66.2 ms, 8.31 -> 8.56 layers/s);
+    // * the passes without the cost phase issue them after their first LDS
+    //   phase instead of at the top of the iteration (N=34 c64 8.58 -> 8.82
+    //   layers/s, N=30 c64 130.2 -> 131.7, half state 247 -> 250; complex128
+    //   unchanged).
+    constexpr bool SPLITPF = PRE && sizeof(R) == 4 && A_LOG == 4;
+    constexpr bool LATEPF = !COST;
     const int row_lane = (t & 63) % (QT + 1);
     double2 fcur = make_double2(0.0, 0.0), fnext = make_double2(0.0, 0.0);
     if (ROWPF && tile < ntiles)
@@ -378,6 +390,27 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_til
         };
         T w[A];
         // the next tile's loads (issued after the cost phase: few registers live)
+        // registers [j0, j1) of the next tile (part of prefetch)
+        auto prefetch_part = [&](int j0, int j1) {
+            if (ROWPF && j0 == 0 && next < ntiles)
+                fnext = ptab[next * q1 + row_lane];
+            if (!FIRST && next < ntiles) {
+                uint64_t roff[A];
+                reg_offsets<A, NB>(rpos_w, roff);
+                const T *p1 = sv + (tile_base(a, next) + toff);
+#pragma unroll
+                for (int j = 0; j < AH; ++j)
+                    if (j >= j0 && j < j1)
+                        w[j] = p1[roff[j]];
+                if (MIRROR && j1 > AH) {
+                    const T *pm = sv + (tile_base(a, ~next & hmask) + a.mirror_pm - toff);
+#pragma unroll
+                    for (int j = AH; j < A; ++j)
+                        if (j >= j0 && j < j1)
+                            w[j] = pm[-(int64_t)roff[j - AH]];
+                }
+            }
+        };
         auto prefetch = [&]() {
             if (ROWPF && next < ntiles)
                 fnext = ptab[next * q1 + row_lane];
@@ -472,8 +505,13 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_til
                     phase_write(eg);
                 }
             }
+            if constexpr (SPLITPF)
+                prefetch_part(0, A / 2);
             apply_cost();
-            prefetch();
+            if constexpr (SPLITPF)
+                prefetch_part(A / 2, A);
+            else
+                prefetch();
             if (G > 0) {
                 rx_registers_t<T, R, A_LOG>(v, group_gates(a, G - 1), rt);
                 eg = phase_slots(G - 1);
@@ -494,7 +532,17 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_til
                 mix_natural<T, R, A_LOG>(v, a, etp, nat, nthr, (R)a.pre_t);
             if (COST)
                 apply_cost();
-            prefetch();
+            if (!LATEPF)
+                prefetch();
+            // LATEPF: after the first LDS phase's write-back, or after the
+            // RX where that phase is stored directly or there is none
+            bool pf_left = LATEPF;
+            auto prefetch_rest = [&]() {
+                if (LATEPF && pf_left) {
+                    prefetch();
+                    pf_left = false;
+                }
+            };
             if (GATES) {
                 const R rt = sizeof(R) == 4 ? (R)a.rx_tf : (R)a.rx_t;
                 rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, rt);
@@ -509,6 +557,8 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_til
                         if (direct && g == a.ngroups - 1)
                             break;
                         phase_write(eg);
+                        if (g == 0)
+                            prefetch_rest();
                     }
                     if (direct) {
                         // every thread has read its last-phase elements before
@@ -519,6 +569,7 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256) k_til
                     }
                 }
             }
+            prefetch_rest();
         }
 
         if (!COST && a.apply_scale) {
