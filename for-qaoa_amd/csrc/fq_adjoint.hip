@@ -177,13 +177,21 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 frow = ptab[tile * q1 + lane];
             }
         }
-        if (PREFETCH && tile + 1 < end_tile) {
-            load_tile(tile + 1, wp, wl);
-            if (COSTBW && row_lane) {
-                cnext = ctab[(tile + 1) * q1 + lane];
-                fnext = ptab[(tile + 1) * q1 + lane];
+        // the next tile's loads go out after the first LDS phase (after the
+        // register RX where there is none), not right after the previous
+        // tile's stores: measured on MI355X, N=26 P=8 complex64 adjoint
+        // 109.3 -> 112.8 gradients/s (pair passes 3.77 -> 3.93 TB/s)
+        bool pf_left = PREFETCH && tile + 1 < end_tile;
+        auto prefetch = [&]() {
+            if (pf_left) {
+                load_tile(tile + 1, wp, wl);
+                if (COSTBW && row_lane) {
+                    cnext = ctab[(tile + 1) * q1 + lane];
+                    fnext = ptab[(tile + 1) * q1 + lane];
+                }
+                pf_left = false;
             }
-        }
+        };
         rx_both<T, R, A_LOG>(vp, vl, a.gate_mask >> nthr, c, s, accX);
         const int nat = lds_pad(t);
         if (a.ngroups > 0) {
@@ -226,7 +234,10 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 ldsL[etp + ej[j]] = vl[j];
             }
             lds_barrier();
+            if (g == 0)
+                prefetch();
         }
+        prefetch();
         if (a.ngroups > 0) {
 #pragma unroll
             for (int j = 0; j < A; ++j) {
