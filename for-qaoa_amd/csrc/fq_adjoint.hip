@@ -71,7 +71,12 @@ __device__ __forceinline__ void rx_both(T *vp, T *vl, unsigned mask, R c, R s, d
 // tile ~tau at complemented offsets (psi(b) = psi(~b)), as in k_tile.  Every
 // element then stands for itself and its complement: the caller doubles the
 // sums.
-template <typename R, int A_LOG, bool COSTBW, bool MIRROR>
+// BOUNDARY (with COSTBW): the pass that undoes a layer-boundary pass of the
+// pipelined forward (RX(beta_p), cost phase gamma_{p+1}, RX(beta_{p+1}) on its
+// qubits): RX(rx angle) = RX(-beta_{p+1}), the conjugate phase, then RX with
+// the coefficients a.pre_c, a.pre_s = RX(-beta_p) on the same gate bits,
+// whose Im<lambda|X_j|psi> sum goes to partials[2*(gridDim.x + block)].
+template <typename R, int A_LOG, bool COSTBW, bool MIRROR, bool BOUNDARY = false>
 __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict__ psi,
                                                     typename CT<R>::T *__restrict__ lam,
                                                     const TileArgs a,
@@ -82,12 +87,13 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     typedef typename CT<R>::T T;
     constexpr int A = 1 << A_LOG;
     extern __shared__ __align__(16) unsigned char fq_smem_a[];
-    __shared__ double s_red[2][8];
+    __shared__ double s_red[3][8];
 
     const int Q = a.Q, nthr = a.nthr;
     const int t = threadIdx.x;
     const int nthreads = blockDim.x;
     static_assert(!(COSTBW && MIRROR), "the cost pass is never the mirror pass");
+    static_assert(!BOUNDARY || COSTBW, "a boundary pair pass has the cost phase");
     constexpr int AH = MIRROR ? A / 2 : A;
     const uint64_t ntiles = run_tiles(a);
     const uint64_t hmask = (1ull << a.nh) - 1ull;
@@ -151,7 +157,7 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     double2 frow = make_double2(1.0, 0.0), fnext = make_double2(1.0, 0.0);
     const bool row_lane = rowpf && lane <= Q;
 
-    double accX = 0.0, accG = 0.0;
+    double accX = 0.0, accG = 0.0, accX2 = 0.0;
     // one contiguous range of tiles per work-group; the next tile's
     // amplitudes load while the current one is in its LDS phases (not in the
     // complex128 cost pass, whose registers are full without it)
@@ -192,8 +198,14 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 pf_left = false;
             }
         };
-        rx_both<T, R, A_LOG>(vp, vl, a.gate_mask >> nthr, c, s, accX);
         const int nat = lds_pad(t);
+        // RX(cc, ss) on every gate bit of both states, its pair sums into
+        // acc: the register bits, then the LDS phases (the next tile's loads
+        // after the first one when `loads`); returns in the natural layout.
+        // Each thread's natural slots are its own, so a second set needs no
+        // barrier before its first write.
+        auto rx_set = [&](R cc, R ss, double &acc, bool loads) {
+        rx_both<T, R, A_LOG>(vp, vl, a.gate_mask >> nthr, cc, ss, acc);
         if (a.ngroups > 0) {
 #pragma unroll
             for (int j = 0; j < A; ++j) {
@@ -227,17 +239,18 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 vp[j] = ldsP[etp + ej[j]];
                 vl[j] = ldsL[etp + ej[j]];
             }
-            rx_both<T, R, A_LOG>(vp, vl, group_gates(a, g), c, s, accX);
+            rx_both<T, R, A_LOG>(vp, vl, group_gates(a, g), cc, ss, acc);
 #pragma unroll
             for (int j = 0; j < A; ++j) {
                 ldsP[etp + ej[j]] = vp[j];
                 ldsL[etp + ej[j]] = vl[j];
             }
             lds_barrier();
-            if (g == 0)
+            if (loads && g == 0)
                 prefetch();
         }
-        prefetch();
+        if (loads)
+            prefetch();
         if (a.ngroups > 0) {
 #pragma unroll
             for (int j = 0; j < A; ++j) {
@@ -245,6 +258,8 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 vl[j] = ldsL[nat + lds_pad(j << nthr)];
             }
         }
+        };
+        rx_set(c, s, accX, true);
         if (COSTBW) {
             const double *c_t = ctab + tile * q1;
             const double2 *f_t = ptab + tile * q1;
@@ -310,6 +325,8 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
                 }
             }
         }
+        if constexpr (BOUNDARY)
+            rx_set((R)a.pre_c, (R)a.pre_s, accX2, false);
         {
             uint64_t roff[A];
             reg_offsets<A, NB>(rpos_w, roff);
@@ -345,21 +362,29 @@ __global__ void __launch_bounds__(512) k_pair_pass(typename CT<R>::T *__restrict
     for (int off = 32; off >= 1; off >>= 1) {
         accX += __shfl_xor(accX, off, 64);
         accG += __shfl_xor(accG, off, 64);
+        if (BOUNDARY)
+            accX2 += __shfl_xor(accX2, off, 64);
     }
     const int wave = t >> 6;
     if (lane == 0) {
         s_red[0][wave] = accX;
         s_red[1][wave] = accG;
+        s_red[2][wave] = accX2;
     }
     __syncthreads();
     if (t == 0) {
-        double x = 0.0, g = 0.0;
+        double x = 0.0, g = 0.0, x2 = 0.0;
         for (int w = 0; w < (nthreads + 63) / 64; ++w) {
             x += s_red[0][w];
             g += s_red[1][w];
+            x2 += s_red[2][w];
         }
         partials[2 * blockIdx.x] = x;
         partials[2 * blockIdx.x + 1] = g;
+        if (BOUNDARY) {
+            partials[2 * (gridDim.x + blockIdx.x)] = x2;
+            partials[2 * (gridDim.x + blockIdx.x) + 1] = 0.0;
+        }
     }
 }
 
@@ -435,14 +460,14 @@ size_t pair_lds_bytes(int dtype, int Q)
 
 // Persistent grid: as many work-groups as can be resident (one partial sum
 // pair each, at most the tile count).
-template <typename R, int A_LOG, bool COSTBW, bool MIRROR>
+template <typename R, int A_LOG, bool COSTBW, bool MIRROR, bool BOUNDARY>
 static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const double2 *ptab,
                                 const double *ctab, double *partials, unsigned *grid_out,
                                 hipStream_t stream)
 {
     typedef typename CT<R>::T T;
     const size_t lds = pair_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q);
-    const void *fn = reinterpret_cast<const void *>(&k_pair_pass<R, A_LOG, COSTBW, MIRROR>);
+    const void *fn = reinterpret_cast<const void *>(&k_pair_pass<R, A_LOG, COSTBW, MIRROR, BOUNDARY>);
     if (lds > 65536) {
         if (hipError_t e = allow_large_lds(fn))
             return e;
@@ -453,21 +478,22 @@ static hipError_t launch_pair_t(void *psi, void *lam, const TileArgs &a, const d
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)device_cu_count();
     const unsigned grid = (unsigned)(tiles < resident ? tiles : resident);
     *grid_out = grid;
-    hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW, MIRROR>), dim3(grid), dim3(threads), lds, stream,
+    hipLaunchKernelGGL((k_pair_pass<R, A_LOG, COSTBW, MIRROR, BOUNDARY>), dim3(grid), dim3(threads), lds, stream,
                        reinterpret_cast<T *>(psi), reinterpret_cast<T *>(lam), a, ptab, ctab, partials);
     return hipGetLastError();
 }
 
-hipError_t launch_pair(int dtype, int a_log, bool costbw, bool mirror, void *psi, void *lam, const TileArgs &a,
-                       const double2 *ptab, const double *ctab, double *partials, unsigned *grid,
-                       hipStream_t stream)
+hipError_t launch_pair(int dtype, int a_log, bool costbw, bool mirror, bool boundary, void *psi, void *lam,
+                       const TileArgs &a, const double2 *ptab, const double *ctab, double *partials,
+                       unsigned *grid, hipStream_t stream)
 {
-    if (costbw && mirror)
+    if ((costbw && mirror) || (boundary && !costbw))
         return hipErrorInvalidValue;
-#define FQ_PAIR(R, AL)                                                                                   \
-    return costbw ? launch_pair_t<R, AL, true, false>(psi, lam, a, ptab, ctab, partials, grid, stream)  \
-           : mirror ? launch_pair_t<R, AL, false, true>(psi, lam, a, ptab, ctab, partials, grid, stream) \
-                    : launch_pair_t<R, AL, false, false>(psi, lam, a, ptab, ctab, partials, grid, stream);
+#define FQ_PAIR(R, AL)                                                                                        \
+    return boundary ? launch_pair_t<R, AL, true, false, true>(psi, lam, a, ptab, ctab, partials, grid, stream)  \
+           : costbw ? launch_pair_t<R, AL, true, false, false>(psi, lam, a, ptab, ctab, partials, grid, stream) \
+           : mirror ? launch_pair_t<R, AL, false, true, false>(psi, lam, a, ptab, ctab, partials, grid, stream) \
+                    : launch_pair_t<R, AL, false, false, false>(psi, lam, a, ptab, ctab, partials, grid, stream);
     if (dtype == 0) {
         switch (a_log) {
             case 1: FQ_PAIR(double, 1)

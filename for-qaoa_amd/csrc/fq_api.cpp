@@ -1861,9 +1861,47 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
     // 2^12-amplitude tiles of psi and lambda share the LDS (2 x 68 KiB);
     // 512 threads x 8 amplitudes of each state
     const int Qb = std::min(12, z2 ? N - 1 : N);
-    const std::vector<FusedPass> ps = cover_passes(N, Qb, st->dtype);
-    const size_t npass = ps.size();
-    if (int rc = ensure_aux(st, 1 + npass * (size_t)P))
+    // The passes of a layer and, as in fq_run's pipelined layers, the pass
+    // besides pass 0 that ends every other layer (end1 < 0: layer order)
+    std::vector<FusedPass> ps = cover_passes(N, Qb, st->dtype);
+    const int m = (int)ps.size();
+    const bool pipe = layer_pipe() && m >= (z2 ? 3 : 2);
+    const int end1 = pipe ? (z2 ? 1 : m - 1) : -1;
+    if (pipe && m >= 3)
+        ps = cover_passes(N, Qb, st->dtype, end1);
+    // The walk back undoes, in reverse order, the launches of a forward run
+    // over this plan: kind 1 a cost pass of layer p (phase, then RX(beta_p)
+    // on its qubits: undone as RX then phase), kind 2 RX(beta_p) alone, kind
+    // 3 a layer-boundary pass (RX(beta_p), phase gamma_{p+1}, RX(beta_{p+1})).
+    // The unitary is the same whichever decomposition the forward used.
+    struct Launch {
+        int pass, kind, p;
+    };
+    std::vector<Launch> fwd;
+    for (int p = 0; p < P; ++p) {
+        if (!pipe) {
+            for (int i = 0; i < m; ++i)
+                fwd.push_back({i, i == 0 ? 1 : 2, p});
+            continue;
+        }
+        const bool last = p + 1 == P;
+        const int e = p % 2 == 0 ? end1 : 0;
+        if (p == 0)
+            fwd.push_back({0, 1, 0});
+        if (last)
+            fwd.push_back({e, 2, p});
+        for (int i = 1; i < m; ++i)
+            if (i != end1)
+                fwd.push_back({i, 2, p});
+        if (!last)
+            fwd.push_back({e, 3, p});
+    }
+    // gradient slots (two doubles each): slot 0 the <C> sums, then per launch
+    // one slot (beta sum, gamma sum), a boundary pass a second (beta_p sum)
+    size_t nslot = 1;
+    for (const Launch &L : fwd)
+        nslot += L.kind == 3 ? 2 : 1;
+    if (int rc = ensure_aux(st, nslot))
         return rc;
 
     // lambda = Cut * psi_P, <C> into slot 0
@@ -1880,53 +1918,78 @@ extern "C" int fq_backward(fq_state *st, fq_graph *g, int P, const double *gamma
         FQ_HIP(launch_cut_mul(st->dtype, a_log, st->sv, st->aux, a, st->d_part, st->stream));
         FQ_HIP(launch_finalize(st->d_part, run_tiles(a), st->d_grad, st->stream));
     }
-    for (int k = P - 1; k >= 0; --k) {
-        for (int i = (int)npass - 1; i >= 0; --i) {
-            const bool costbw = i == 0;
-            const int Q = (int)ps[i].tile.size();
-            // 512 threads x 8 amplitudes of each state
-            const int a_log = std::max(1, std::min(3, Q));
-            const bool mirror = z2 && i + 1 == (int)npass;
-            Geom geom;
-            TileArgs a;
-            if (int rc = pass_args(N, st->dtype, ps[i].tile, ps[i].gates, costbw,
-                                   z2 ? (mirror ? Z2_MIRROR : Z2_LOWER) : Z2_OFF, a, geom, a_log))
-                return rc;
-            rx_coeffs(-betas[k], st->dtype, &a.rx_c, &a.rx_s);
-            double *ctab = nullptr;
-            if (costbw) {
-                if (int rc = get_plan(g, geom, &a.plan, st->stream))
-                    return rc;
-                set_cost(a, -gammas[k], st->init_amp);
-                if (int rc = prepare_phases(st, a, a_log, &ctab))
-                    return rc;
-            }
-            double *slot = st->d_grad + 2 * (1 + (size_t)k * npass + i);
-            int rc = timed(st, FQ_K_ADJOINT, 4.0 * (double)st->bytes / mult, [&] {
-                unsigned grid = 0;
-                hipError_t e = launch_pair(st->dtype, a_log, costbw, mirror, st->sv, st->aux, a,
-                                           costbw ? st->d_phase : nullptr, ctab, st->d_part, &grid,
-                                           st->stream);
-                if (e != hipSuccess)
-                    return e;
-                return launch_finalize(st->d_part, grid, slot, st->stream);
-            });
-            if (rc)
-                return rc;
+    // (launch index in fwd, first slot) of every launch, for the sums below
+    std::vector<size_t> slot_of(fwd.size());
+    {
+        size_t s_next = 1;
+        for (size_t n = 0; n < fwd.size(); ++n) {
+            slot_of[n] = s_next;
+            s_next += fwd[n].kind == 3 ? 2 : 1;
         }
     }
-    std::vector<double> h(2 * (1 + npass * (size_t)P));
+    for (size_t n = fwd.size(); n-- > 0;) {
+        const Launch &L = fwd[n];
+        const int i = L.pass;
+        const bool costbw = L.kind != 2;
+        const bool boundary = L.kind == 3;
+        const int Q = (int)ps[i].tile.size();
+        // 512 threads x 8 amplitudes of each state
+        const int a_log = std::max(1, std::min(3, Q));
+        const bool mirror = z2 && i + 1 == m;
+        Geom geom;
+        TileArgs a;
+        if (int rc = pass_args(N, st->dtype, ps[i].tile, ps[i].gates, costbw,
+                               z2 ? (mirror ? Z2_MIRROR : Z2_LOWER) : Z2_OFF, a, geom, a_log))
+            return rc;
+        // the RX undone first: beta_{p+1} in a boundary pass, else beta_p
+        const int pb = boundary ? L.p + 1 : L.p;
+        rx_coeffs(-betas[pb], st->dtype, &a.rx_c, &a.rx_s);
+        if (boundary)
+            rx_coeffs(-betas[L.p], st->dtype, &a.pre_c, &a.pre_s);
+        double *ctab = nullptr;
+        if (costbw) {
+            if (int rc = get_plan(g, geom, &a.plan, st->stream))
+                return rc;
+            set_cost(a, -gammas[pb], st->init_amp);
+            if (int rc = prepare_phases(st, a, a_log, &ctab))
+                return rc;
+        }
+        if (boundary && 2 * run_tiles(a) > st->part_cap)
+            return set_err(FQ_ERR_ARG, "internal: partial-sum buffer too small for a boundary pair pass");
+        double *slot = st->d_grad + 2 * slot_of[n];
+        int rc = timed(st, FQ_K_ADJOINT, 4.0 * (double)st->bytes / mult, [&] {
+            unsigned grid = 0;
+            hipError_t e = launch_pair(st->dtype, a_log, costbw, mirror, boundary, st->sv, st->aux, a,
+                                       costbw ? st->d_phase : nullptr, ctab, st->d_part, &grid, st->stream);
+            if (e != hipSuccess)
+                return e;
+            e = launch_finalize(st->d_part, grid, slot, st->stream);
+            if (e != hipSuccess || !boundary)
+                return e;
+            return launch_finalize(st->d_part + 2 * (size_t)grid, grid, slot + 2, st->stream);
+        });
+        if (rc)
+            return rc;
+    }
+    std::vector<double> h(2 * nslot);
     FQ_HIP(hipMemcpyAsync(h.data(), st->d_grad, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
                           st->stream));
     FQ_HIP(hipStreamSynchronize(st->stream));
     if (expect)
         *expect = mult * h[0];
     for (int k = 0; k < P; ++k) {
-        double gb = 0.0;
-        for (size_t i = 0; i < npass; ++i)
-            gb += h[2 * (1 + k * npass + i)];
-        grad_betas[k] = mult * gb;
-        grad_gammas[k] = -mult * h[2 * (1 + k * npass) + 1];
+        grad_betas[k] = 0.0;
+        grad_gammas[k] = 0.0;
+    }
+    for (size_t n = 0; n < fwd.size(); ++n) {
+        const Launch &L = fwd[n];
+        const size_t s0 = slot_of[n];
+        const int pb = L.kind == 3 ? L.p + 1 : L.p;
+        grad_betas[pb] += mult * h[2 * s0];
+        if (L.kind != 2)
+            grad_gammas[pb] = -mult * h[2 * s0 + 1];
+        if (L.kind == 3)
+            grad_betas[L.p] += mult * h[2 * (s0 + 1)];
     }
     return FQ_OK;
 }

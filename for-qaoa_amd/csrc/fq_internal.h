@@ -159,9 +159,12 @@ hipError_t launch_finalize(const double *partials, uint64_t n, double *out2, hip
 // With costbw, ptab and ctab are the phase and cut tables of one
 // launch_phase_table call.
 // mirror: the half-state mirror pass (qubit N-1 the top register bit).
-hipError_t launch_pair(int dtype, int a_log, bool costbw, bool mirror, void *psi, void *lam, const TileArgs &a,
-                       const double2 *ptab, const double *ctab, double *partials, unsigned *grid,
-                       hipStream_t stream);
+// boundary (with costbw): after the conjugate phase, RX with a.pre_c, a.pre_s
+// on the same gate bits; its sums go to partials[2 * (*grid) ...], the
+// work-group's second pair of partial sums.
+hipError_t launch_pair(int dtype, int a_log, bool costbw, bool mirror, bool boundary, void *psi, void *lam,
+                       const TileArgs &a, const double2 *ptab, const double *ctab, double *partials,
+                       unsigned *grid, hipStream_t stream);
 hipError_t launch_cut_mul(int dtype, int a_log, const void *psi, void *lam, const TileArgs &a,
                           double *partials, hipStream_t stream);
 size_t pair_lds_bytes(int dtype, int Q);

@@ -177,3 +177,44 @@ def test_adjoint_random_configurations(case):
     finally:
         st.destroy()
         gr.destroy()
+
+
+@pytest.mark.parametrize("N,P", [(21, 2), (22, 3), (24, 4), (23, 5)])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_pipelined_walk_back_matches_layer_order(N, P, dtype, monkeypatch):
+    # From 3 pair passes per layer on, the walk back undoes the launches of a
+    # pipelined forward: layer-boundary pair passes undo RX(beta_{p+1}), the
+    # phase gamma_{p+1} and RX(beta_p) in one launch.  The gradients equal the
+    # layer-order walk back's (FQ_LAYER_PIPE=0) and central differences.
+    edges = G.gnp(N, 0.4, 700 + N, "uniform")
+    g, b = G.angles(P, 710 + N)
+    gr = Graph(N, edges)
+    st = State(N, C=min(N, 13 if dtype == "c128" else 14), dtype=dtype)
+    try:
+        st.run(gr, g, b, "z2")
+        e1, gg1, gb1 = st.backward(gr, g, b)
+        monkeypatch.setenv("FQ_LAYER_PIPE", "0")
+        st.run(gr, g, b, "z2")
+        e0, gg0, gb0 = st.backward(gr, g, b)
+        monkeypatch.delenv("FQ_LAYER_PIPE")
+        ref = State(N, C=13, dtype="c128")
+
+        def f(x, y):
+            ref.run(gr, x, y, "fused")
+            return ref.expect_cut(gr)[0]
+
+        try:
+            fg, fb = central_fd(f, g, b, 1e-4)
+        finally:
+            ref.destroy()
+    finally:
+        st.destroy()
+        gr.destroy()
+    scale = max(1.0, float(np.max(np.abs(np.concatenate([gg0, gb0])))))
+    d_order = float(np.max(np.abs(np.concatenate([gg1 - gg0, gb1 - gb0])))) / scale
+    d_fd = float(np.max(np.abs(np.concatenate([gg1 - fg, gb1 - fb])))) / scale
+    print(f"walk back N={N} P={P} {dtype}: pipelined vs layer order {d_order:.2e}, "
+          f"vs central FD {d_fd:.2e}", flush=True)
+    assert e1 == pytest.approx(e0, rel=1e-12 if dtype == "c128" else 1e-5)
+    assert d_order < (1e-12 if dtype == "c128" else 1e-5)
+    assert d_fd < (1e-7 if dtype == "c128" else 1e-4)
