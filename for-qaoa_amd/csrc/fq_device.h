@@ -75,6 +75,22 @@ __device__ __forceinline__ void rx_registers_t(T *v, unsigned mask, R t)
     }
 }
 
+// RX on register bits 0..K-1 only (compile time): the last LDS phase of a
+// pass whose gated bits are its K lowest register bits, the others filling
+// the phase, runs no identity butterflies on the fillers.
+template <typename T, typename R, int A_LOG, int K>
+__device__ __forceinline__ void rx_registers_low(T *v, R t)
+{
+    constexpr int A = 1 << A_LOG;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+#pragma unroll
+        for (int j = 0; j < A; ++j)
+            if (!((j >> r) & 1))
+                rx_pair_t<T, R>(v[j], v[j | (1 << r)], t);
+    }
+}
+
 // v *= (c + i s)
 template <typename T, typename R>
 __device__ __forceinline__ void cmul_by(T &v, R c, R s)

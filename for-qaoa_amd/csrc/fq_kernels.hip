@@ -204,7 +204,11 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
 // passes that left theirs pending) as a.scale_re + i a.scale_im, which a cost
 // pass folds into its FP64 phase factor and a pass without the cost phase
 // multiplies in before its stores when a.apply_scale is set.
-template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false>
+// KL > 0: the last LDS phase's gated bits are its KL lowest register bits
+// (the host launches this instance only then), and its RX runs on those
+// alone instead of running identity butterflies on the filler bits.
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false,
+          int KL = 0>
 __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
     k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a, const double2 *__restrict__ ptab)
 {
@@ -463,6 +467,16 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
                 lds_opaque(eg);
             rx_registers_t<T, R, A_LOG>(v, group_gates(a, g), tt);
         };
+        // the last LDS phase of a KL instance
+        auto phase_read_last = [&](int g, int &eg, R tt) {
+            eg = phase_slots(g);
+#pragma unroll
+            for (int j = 0; j < A; ++j)
+                v[j] = lds[eg + ej[j]];
+            if (COST && sizeof(R) == 4)
+                lds_opaque(eg);
+            rx_registers_low<T, R, A_LOG, (KL > 0 ? KL : A_LOG)>(v, tt);
+        };
         auto phase_write = [&](int eg) {
 #pragma unroll
             for (int j = 0; j < A; ++j)
@@ -497,12 +511,21 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
             int eg = 0;
             if (G > 0) {
                 natural_write();
+                if constexpr (KL > 0) {
 #pragma unroll 1
-                for (int g = 0; g < G; ++g) {
-                    phase_read_rx(g, eg, pt);
-                    if (g + 1 == G)
-                        break;
-                    phase_write(eg);
+                    for (int g = 0; g + 1 < G; ++g) {
+                        phase_read_rx(g, eg, pt);
+                        phase_write(eg);
+                    }
+                    phase_read_last(G - 1, eg, pt);
+                } else {
+#pragma unroll 1
+                    for (int g = 0; g < G; ++g) {
+                        phase_read_rx(g, eg, pt);
+                        if (g + 1 == G)
+                            break;
+                        phase_write(eg);
+                    }
                 }
             }
             if constexpr (SPLITPF)
@@ -513,7 +536,10 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
             else
                 prefetch();
             if (G > 0) {
-                rx_registers_t<T, R, A_LOG>(v, group_gates(a, G - 1), rt);
+                if constexpr (KL > 0)
+                    rx_registers_low<T, R, A_LOG, KL>(v, rt);
+                else
+                    rx_registers_t<T, R, A_LOG>(v, group_gates(a, G - 1), rt);
                 eg = phase_slots(G - 1);
                 phase_write(eg);
 #pragma unroll 1
@@ -551,14 +577,30 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
                     // rolled loop: unrolling lets the compiler hoist every
                     // phase's addresses out of the tile loop and spill
                     int eg = 0;
+                    if constexpr (KL > 0) {
+                        const int G = a.ngroups;
 #pragma unroll 1
-                    for (int g = 0; g < a.ngroups; ++g) {
-                        phase_read_rx(g, eg, rt);
-                        if (direct && g == a.ngroups - 1)
-                            break;
-                        phase_write(eg);
-                        if (g == 0)
-                            prefetch_rest();
+                        for (int g = 0; g + 1 < G; ++g) {
+                            phase_read_rx(g, eg, rt);
+                            phase_write(eg);
+                            if (g == 0)
+                                prefetch_rest();
+                        }
+                        // (KL instances run with two or more phases: the
+                        // loads went out after the first)
+                        phase_read_last(G - 1, eg, rt);
+                        if (!direct)
+                            phase_write(eg);
+                    } else {
+#pragma unroll 1
+                        for (int g = 0; g < a.ngroups; ++g) {
+                            phase_read_rx(g, eg, rt);
+                            if (direct && g == a.ngroups - 1)
+                                break;
+                            phase_write(eg);
+                            if (g == 0)
+                                prefetch_rest();
+                        }
                     }
                     if (direct) {
                         // every thread has read its last-phase elements before
@@ -1103,6 +1145,7 @@ int device_cu_count()
 
 static int device_cus() { return device_cu_count(); }
 
+
 hipError_t allow_large_lds(const void *fn)
 {
     static std::mutex mu;
@@ -1146,14 +1189,14 @@ int occupancy_per_cu(const void *fn, int threads, size_t lds)
     return per_cu;
 }
 
-template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, bool PRE = false>
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, bool PRE = false, int KL = 0>
 static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab, hipStream_t stream)
 {
     typedef typename CT<R>::T T;
     const unsigned threads = 1u << a.nthr;
     const uint64_t tiles = run_tiles(a);
     const size_t lds = GATES && a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
-    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE>);
+    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE, KL>);
     if (lds > 65536) {
         if (hipError_t e = allow_large_lds(fn))
             return e;
@@ -1166,7 +1209,7 @@ static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab
     TileArgs la = a;
     if (la.order == 1 && grid % 8 != 0)
         la.order = 0;
-    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE>), dim3((unsigned)grid), dim3(threads),
+    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE, KL>), dim3((unsigned)grid), dim3(threads),
                        lds, stream, reinterpret_cast<T *>(sv), la, ptab);
     return hipGetLastError();
 }
@@ -1237,13 +1280,26 @@ static hipError_t launch_cost_only(bool first, void *sv, const TileArgs &a, cons
                  : launch_tile_t<R, A_LOG, QT, true, false, false>(sv, a, ptab, stream);
 }
 
+// The last LDS phase gates its two lowest register bits only (the complex64
+// strided passes' 14 = 4 + 4 + 4 + 2 and 10 = 4 + 4 + 2 gates): the KL = 2
+// instances skip the butterflies of its two filler bits (N=34 complex64:
+// strided boundary pass 67.9 -> 64.8 ms, plain strided pass 52.5 -> 50.8 ms).
+static bool last_phase_low2(const TileArgs &a)
+{
+    return a.ngroups >= 2 && a.grp_gates[a.ngroups - 1] == 3u;
+}
+
 // Gated passes, with or without the cost phase.
 template <typename R, int A_LOG, int QT>
 static hipError_t launch_gated(bool cost, bool first, void *sv, const TileArgs &a, const double2 *ptab,
                                hipStream_t stream)
 {
-    if (!cost)
+    if (!cost) {
+        if constexpr (sizeof(R) == 4 && A_LOG == 4 && QT == 14)
+            if (last_phase_low2(a))
+                return launch_tile_t<R, A_LOG, QT, false, false, true, false, 2>(sv, a, ptab, stream);
         return launch_tile_t<R, A_LOG, QT, false, false, true>(sv, a, ptab, stream);
+    }
     return first ? launch_tile_t<R, A_LOG, QT, true, true, true>(sv, a, ptab, stream)
                  : launch_tile_t<R, A_LOG, QT, true, false, true>(sv, a, ptab, stream);
 }
@@ -1316,7 +1372,8 @@ static hipError_t launch_pre_r(int a_log, void *sv, const TileArgs &a, const dou
         if (a_log == 5 && a.Q == 14)
             return launch_tile_t<R, 5, 14, true, false, true, true>(sv, a, ptab, stream);
         if (a_log == 4 && a.Q == 14)
-            return launch_tile_t<R, 4, 14, true, false, true, true>(sv, a, ptab, stream);
+            return last_phase_low2(a) ? launch_tile_t<R, 4, 14, true, false, true, true, 2>(sv, a, ptab, stream)
+                                      : launch_tile_t<R, 4, 14, true, false, true, true>(sv, a, ptab, stream);
         if (a_log == 4 && a.Q == 13)
             return launch_tile_t<R, 4, 13, true, false, true, true>(sv, a, ptab, stream);
     }
