@@ -430,15 +430,19 @@ static std::string geom_key(const Geom &g)
 }
 
 // RX groups: gate bits among the thread bits, a_log per LDS phase.
-static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask)
+// lane_swap: thread bits 4 and 5 (gated) are exchanged with register bits 1
+// and 0 by cross-lane swaps and get their RX there; the LDS phases take the
+// other gated thread bits.
+static int set_gates(TileArgs &a, const Geom &g, uint32_t gate_mask, bool lane_swap = false)
 {
     a.gate_mask = gate_mask;
+    a.lane_swap = lane_swap ? 1u : 0u;
     // gated thread bits, lowest first: the last LDS phase gets the highest
     // ones, so that it usually holds none of the six lane bits and the tile
     // can be stored straight from its layout (direct_store)
     std::vector<int> pend;
     for (int i = 0; i < g.nthr; ++i)
-        if ((gate_mask >> i) & 1u)
+        if (((gate_mask >> i) & 1u) && !(lane_swap && (i == 4 || i == 5)))
             pend.push_back(i);
     a.ngroups = 0;
     a.grp_mask_w = 0;
@@ -1149,6 +1153,17 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
     if (int rc = pass_args(st->N, st->dtype, tile, gates, cost, z2, a, geom))
         return rc;
     const int a_log = geom.a_log;
+    // complex64 layer-boundary passes on 1024 x 16 tiles whose lane bits 4
+    // and 5 are gated: RX on those two by swaps between lanes and registers,
+    // one LDS phase fewer (at least one must remain: the pass applies its
+    // cost phase in the last phase's layout)
+    if (st->dtype == FQ_C64 && a_log == 4 && geom.Q == 14 && z2 < Z2_MIRROR && pre_beta &&
+        ((a.gate_mask >> 4) & 3u) == 3u) {
+        uint32_t rest = a.gate_mask & ((1u << geom.nthr) - 1u) & ~0x30u;
+        if (rest != 0)
+            if (int rc = set_gates(a, geom, a.gate_mask, true))
+                return rc;
+    }
     // the pass's factor: sigma^(#gates) per RX set, times what is pending
     double sr = st->pending_re, si = st->pending_im;
     auto absorb = [&](double b, double *t) {

@@ -91,6 +91,25 @@ __device__ __forceinline__ void rx_registers_low(T *v, R t)
     }
 }
 
+// Exchange of a wave's lanes 32..63 of a with lanes 0..31 of b
+// (v_permlane32_swap): for a pair of registers a = v[j], b = v[j | 2^r]
+// this exchanges lane bit 5 with register bit r.
+__device__ __forceinline__ void lane_xchg32(float &a, float &b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+// The same for lane bit 4 (v_permlane16_swap: the odd 16-lane rows of a
+// with the even rows of b).
+__device__ __forceinline__ void lane_xchg16(float &a, float &b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
 // v *= (c + i s)
 template <typename T, typename R>
 __device__ __forceinline__ void cmul_by(T &v, R c, R s)

@@ -70,6 +70,7 @@ struct TileArgs {
     uint64_t grp_mask_w;     // grp_mask[0..3], 16 bits each (read with a shift, not an index)
     uint32_t grp_gates_w;    // grp_gates[0..4], 5 bits each
     uint32_t direct_store;   // last LDS phase holds no lane bit: store from its layout
+    uint32_t lane_swap;      // RX on thread bits 4 and 5 by cross-lane swaps, not in an LDS phase
     uint32_t dpos_w;         // direct_store: position of the last phase's register bit r (bit 6r)
     double rx_c, rx_s;       // cos(beta/2), sin(beta/2) (host-computed, src/rx.cu:121)
     double pre_c, pre_s;     // layer-boundary pass: RX of the previous layer, applied first

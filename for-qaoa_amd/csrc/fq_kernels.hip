@@ -207,8 +207,18 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
 // KL > 0: the last LDS phase's gated bits are its KL lowest register bits
 // (the host launches this instance only then), and its RX runs on those
 // alone instead of running identity butterflies on the filler bits.
+// LS (the complex64 layer-boundary pass at 1024 x 16, a.lane_swap): thread
+// bits 4 and 5, both gated, are exchanged with register bits 1 and 0 by
+// v_permlane16_swap / v_permlane32_swap right after the natural layout's RX,
+// get their RX there, and go to the LDS in that exchanged layout; the LDS
+// phases take the other gated thread bits, and after the second RX set the
+// swaps bring the natural layout back for the stores.  At N=34 that turns
+// the strided boundary pass's two LDS phases (4 + 2 bits) into one (strided
+// boundary pass 65.6 -> 58.3 ms).  The plain passes do not use it: with
+// their last phase's registers on other qubits their direct stores measured
+// far slower (N=34 plain strided pass 51.2 -> 75.6 ms).
 template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false,
-          int KL = 0>
+          int KL = 0, bool LS = false>
 __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
     k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a, const double2 *__restrict__ ptab)
 {
@@ -235,7 +245,8 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
     // per-thread LDS slots of every remapping phase (padded), once per launch;
     // lds_pad is additive over disjoint bit sets: pad(x | y) = pad(x) + pad(y)
     int etp[MAXG];
-    const int nat = lds_pad(t);
+    const int nat = LS ? lds_pad((t & ~0x30) | (((t >> 5) & 1) << (QT - A_LOG)) | (((t >> 4) & 1) << (QT - A_LOG + 1)))
+                       : lds_pad(t);
     if (GATES) {
         const unsigned qmask = (1u << Q) - 1u;
 #pragma unroll
@@ -483,18 +494,46 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
                 lds[eg + ej[j]] = v[j];
             lds_barrier();
         };
+        // LS: element (t, j) of the exchanged layout is tile element
+        // (t without bits 4, 5) | t4 << (nthr + 1) | t5 << nthr | j0 << 5 |
+        // j1 << 4 | (j >> 2) << (nthr + 2); lds_pad is additive over disjoint
+        // bits, so the thread's part (nat) and register j's (a constant) add
+        auto reg_slot = [&](int j) -> int {
+            if constexpr (LS)
+                return lds_pad(((j & 1) << 5) | (((j >> 1) & 1) << 4) | ((j >> 2) << (QT - A_LOG + 2)));
+            else
+                return lds_pad(j << nthr);
+        };
         auto natural_write = [&]() {
             // own slots: the previous tile's final read used the same slots,
             // so no barrier is needed before this write
 #pragma unroll
             for (int j = 0; j < A; ++j)
-                lds[nat + lds_pad(j << nthr)] = v[j];
+                lds[nat + reg_slot(j)] = v[j];
             lds_barrier();
         };
         auto natural_read = [&]() {
 #pragma unroll
             for (int j = 0; j < A; ++j)
-                v[j] = lds[nat + lds_pad(j << nthr)];
+                v[j] = lds[nat + reg_slot(j)];
+        };
+        // LS: exchange thread bit 5 with register bit 0 and thread bit 4 with
+        // register bit 1 (each swap is its own inverse, and the two commute)
+        auto lane_swap = [&]() {
+            if constexpr (LS) {
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    if (!(j & 1)) {
+                        lane_xchg32(v[j].x, v[j | 1].x);
+                        lane_xchg32(v[j].y, v[j | 1].y);
+                    }
+#pragma unroll
+                for (int j = 0; j < A; ++j)
+                    if (!(j & 2)) {
+                        lane_xchg16(v[j].x, v[j | 2].x);
+                        lane_xchg16(v[j].y, v[j | 2].y);
+                    }
+            }
         };
 
         if constexpr (PRE && sizeof(R) == 4) {
@@ -508,6 +547,10 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
             const R pt = a.pre_tf, rt = a.rx_tf;
             const int G = a.ngroups;
             rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, pt);
+            if constexpr (LS) {
+                lane_swap();
+                rx_registers_low<T, R, A_LOG, 2>(v, pt);
+            }
             int eg = 0;
             if (G > 0) {
                 natural_write();
@@ -548,6 +591,10 @@ __global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
                     phase_write(eg);
                 }
                 natural_read();
+            }
+            if constexpr (LS) {
+                rx_registers_low<T, R, A_LOG, 2>(v, rt);
+                lane_swap();
             }
             rx_registers_t<T, R, A_LOG>(v, a.gate_mask >> nthr, rt);
         } else {
@@ -1189,14 +1236,15 @@ int occupancy_per_cu(const void *fn, int threads, size_t lds)
     return per_cu;
 }
 
-template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, bool PRE = false, int KL = 0>
+template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, bool PRE = false, int KL = 0,
+          bool LS = false>
 static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab, hipStream_t stream)
 {
     typedef typename CT<R>::T T;
     const unsigned threads = 1u << a.nthr;
     const uint64_t tiles = run_tiles(a);
     const size_t lds = GATES && a.ngroups > 0 ? tile_lds_bytes(sizeof(R) == 8 ? 0 : 1, a.Q) : 0;
-    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE, KL>);
+    const void *fn = reinterpret_cast<const void *>(&k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE, KL, LS>);
     if (lds > 65536) {
         if (hipError_t e = allow_large_lds(fn))
             return e;
@@ -1209,7 +1257,7 @@ static hipError_t launch_tile_t(void *sv, const TileArgs &a, const double2 *ptab
     TileArgs la = a;
     if (la.order == 1 && grid % 8 != 0)
         la.order = 0;
-    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE, KL>), dim3((unsigned)grid), dim3(threads),
+    hipLaunchKernelGGL((k_tile<R, A_LOG, QT, COST, FIRST, GATES, 0, PRE, KL, LS>), dim3((unsigned)grid), dim3(threads),
                        lds, stream, reinterpret_cast<T *>(sv), la, ptab);
     return hipGetLastError();
 }
@@ -1295,6 +1343,8 @@ static hipError_t launch_gated(bool cost, bool first, void *sv, const TileArgs &
                                hipStream_t stream)
 {
     if (!cost) {
+        if (a.lane_swap)
+            return hipErrorInvalidValue;
         if constexpr (sizeof(R) == 4 && A_LOG == 4 && QT == 14)
             if (last_phase_low2(a))
                 return launch_tile_t<R, A_LOG, QT, false, false, true, false, 2>(sv, a, ptab, stream);
@@ -1371,9 +1421,12 @@ static hipError_t launch_pre_r(int a_log, void *sv, const TileArgs &a, const dou
     } else {
         if (a_log == 5 && a.Q == 14)
             return launch_tile_t<R, 5, 14, true, false, true, true>(sv, a, ptab, stream);
-        if (a_log == 4 && a.Q == 14)
+        if (a_log == 4 && a.Q == 14) {
+            if (a.lane_swap)
+                return launch_tile_t<R, 4, 14, true, false, true, true, 0, true>(sv, a, ptab, stream);
             return last_phase_low2(a) ? launch_tile_t<R, 4, 14, true, false, true, true, 2>(sv, a, ptab, stream)
                                       : launch_tile_t<R, 4, 14, true, false, true, true>(sv, a, ptab, stream);
+        }
         if (a_log == 4 && a.Q == 13)
             return launch_tile_t<R, 4, 13, true, false, true, true>(sv, a, ptab, stream);
     }
