@@ -27,12 +27,15 @@ from forqaoa.serve import Slab
 pytestmark = pytest.mark.gpu
 
 
-def test_n34_c64_complete_graph_closed_form():
+@pytest.mark.parametrize("schedule", ["fused", "z2"])
+def test_n34_c64_complete_graph_closed_form(schedule):
+    # (both schedules run the complex64 layer-boundary pass whose lane bits 4
+    # and 5 get their RX through lane swaps)
     c = [k for k in GU.kn_cases() if k["N"] == 34][0]
     N = 34
     st = State(N, C=14, dtype="c64")
     g = Graph(N, G.complete(N))
-    st.run(g, c["gammas"], c["betas"], "fused")
+    st.run(g, c["gammas"], c["betas"], schedule)
     rng = np.random.default_rng(34)
     idx = rng.integers(0, 1 << N, size=8192, dtype=np.uint64)
     got = st.gather(idx, logical=True)
@@ -41,7 +44,7 @@ def test_n34_c64_complete_graph_closed_form():
     err = float(np.max(np.abs(got - want)) / rms)
     ex, n2 = st.expect_cut(g)
     rel = abs(ex - c["expect_cut"]) / c["expect_cut"]
-    print(f"N=34 c64 K_34 P=2: max amplitude error / 2^-17 = {err:.2e}, <C> rel err {rel:.2e}, "
+    print(f"N=34 c64 K_34 P=2 {schedule}: max amplitude error / 2^-17 = {err:.2e}, <C> rel err {rel:.2e}, "
           f"norm2-1 = {n2 - 1:.2e}", flush=True)
     assert err < 1e-4
     assert rel < 1e-5
