@@ -1192,7 +1192,6 @@ int device_cu_count()
 
 static int device_cus() { return device_cu_count(); }
 
-
 hipError_t allow_large_lds(const void *fn)
 {
     static std::mutex mu;
