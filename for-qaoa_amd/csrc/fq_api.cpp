@@ -773,7 +773,13 @@ static int slab_a_log(int dtype, int Q)
 // same number of phases the wider tile only adds filler butterflies and
 // halves the waves per CU (N=30: 130.2 vs 127.9 layers/s), and the strided
 // tiles measured slower with it (N=34 plain strided pass 55.4 -> 60.5 ms).
-static int tile_a_log(int dtype, const std::vector<int> &tile, const std::vector<int> &gates)
+// Only the write-only layer-0 pass (first) keeps it where qubits 4 and 5 are
+// gated: at 1024 x 16 the layer-boundary pass then gets their RX by lane
+// swaps and needs one LDS phase, and the other passes drop the half-filled
+// last phase's filler butterflies; N=34 boundary pass 0 57.3-58.3 -> 50.8 ms,
+// plain pass 0 46.9 -> 45.6 ms, while the layer-0 pass at 1024 x 16 took
+// 35.5 ms against 33.6-34.2.
+static int tile_a_log(int dtype, const std::vector<int> &tile, const std::vector<int> &gates, bool first = false)
 {
     const int Q = (int)tile.size();
     bool low = true;
@@ -782,11 +788,14 @@ static int tile_a_log(int dtype, const std::vector<int> &tile, const std::vector
     if (dtype == FQ_C64 && Q == 14 && low && wide_regs()) {
         // gated thread bits (tile position = qubit here) and the LDS phases they need
         int g4 = 0, g5 = 0;
+        bool l4 = false, l5 = false;
         for (int q : gates) {
             g4 += q < 10;
             g5 += q < 9;
+            l4 = l4 || q == 4;
+            l5 = l5 || q == 5;
         }
-        if ((g5 + 4) / 5 < (g4 + 3) / 4)
+        if ((g5 + 4) / 5 < (g4 + 3) / 4 && (first || !(l4 && l5)))
             return 5;
     }
     return slab_a_log(dtype, Q);
@@ -1097,10 +1106,10 @@ enum { Z2_OFF = 0, Z2_LOWER = 1, Z2_MIRROR = 2, Z2_EXPAND = 3 };
 // Launch arguments of one tile pass of an N-qubit state: tile positions
 // (thread bits first), RX gate positions and the half-state mode.
 static int pass_args(int N, int dtype, const std::vector<int> &tile, const std::vector<int> &gates, bool cost,
-                     int z2, TileArgs &a, Geom &geom, int a_log_override = 0)
+                     int z2, TileArgs &a, Geom &geom, int a_log_override = 0, bool first = false)
 {
     const int Q = (int)tile.size();
-    const int a_log = a_log_override > 0 ? a_log_override : tile_a_log(dtype, tile, gates);
+    const int a_log = a_log_override > 0 ? a_log_override : tile_a_log(dtype, tile, gates, first);
     geom = make_geom(N, tile, a_log);
     if (geom_runs(geom) > FQ_MAX_RUNS)
         return set_err(FQ_ERR_ARG, "internal: tile geometry needs more than %d bit runs", FQ_MAX_RUNS);
@@ -1150,7 +1159,7 @@ static int tile_pass(fq_state *st, const std::vector<int> &tile, const std::vect
         return set_err(FQ_ERR_ARG, "internal: a layer-boundary pass needs the cost phase of a later layer");
     TileArgs a;
     Geom geom;
-    if (int rc = pass_args(st->N, st->dtype, tile, gates, cost, z2, a, geom))
+    if (int rc = pass_args(st->N, st->dtype, tile, gates, cost, z2, a, geom, 0, first))
         return rc;
     const int a_log = geom.a_log;
     // complex64 layer-boundary passes on 1024 x 16 tiles whose lane bits 4
@@ -2458,12 +2467,16 @@ static int slab_make_views(fq_slab *sb)
     for (int i = 0; i < m; ++i) {
         if (i != 0 && i != lp.end1)
             continue;
-        TileArgs a;
-        Geom geom;
-        const int z2 = !sb->z2 ? Z2_OFF : i + 1 < m ? Z2_LOWER : Z2_MIRROR;
-        if (int rc = pass_args(sb->N, sb->dtype, lp.ps[i].tile, lp.ps[i].gates, true, z2, a, geom))
-            return rc;
-        need = std::max(need, phase_table_entries(a, geom.a_log) * sizeof(double2));
+        // pass 0 runs as the write-only layer-0 pass too, possibly with
+        // another register width (tile_a_log)
+        for (int first = 0; first <= (i == 0 ? 1 : 0); ++first) {
+            TileArgs a;
+            Geom geom;
+            const int z2 = !sb->z2 ? Z2_OFF : i + 1 < m ? Z2_LOWER : Z2_MIRROR;
+            if (int rc = pass_args(sb->N, sb->dtype, lp.ps[i].tile, lp.ps[i].gates, true, z2, a, geom, 0, first != 0))
+                return rc;
+            need = std::max(need, phase_table_entries(a, geom.a_log) * sizeof(double2));
+        }
     }
     for (int k = 0; k < sb->K; ++k) {
         fq_state *v = new fq_state();

@@ -47,8 +47,9 @@ CASES = [
     ({"FQ_CARRIER_GATES": "0"}, 24, "c64"),
     ({"FQ_CARRIER_GATES": "0", "FQ_LAYER_PIPE": "0"}, 22, "c64"),
     # 512 x 32 is the default where it saves an LDS phase of the contiguous
-    # pass: at N=23 and 24 (and 32..34) its RX set (qubits 5..13 or 4..13)
-    # takes one phase of five bits instead of two of four
+    # pass: at N=23 (and 32, 33) its RX set (qubits 5..13) takes one phase of
+    # five bits instead of two of four; at N=24 (and 34, RX on 4..13) only the
+    # write-only layer-0 pass keeps it
     ({"FQ_WIDE_REGS": "1"}, 24, "c64"),
     ({"FQ_WIDE_REGS": "1", "FQ_LAYER_PIPE": "0"}, 23, "c64"),
     ({"FQ_WIDE_REGS": "0"}, 24, "c64"),
