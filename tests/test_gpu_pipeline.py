@@ -66,8 +66,10 @@ CASES = [
     # 3 passes per layer (fused from N=24 complex128; half state from N=24)
     (24, "c128", "fused", 3), (24, "c128", "z2", 2), (24, "c128", "z2", 3), (25, "c128", "z2", 4),
     (24, "c128", "fused", 4),
-    # complex64
+    # complex64 (N=24, P=4: the contiguous pass 0 ends layers 1 and 3 as a
+    # 1024 x 16 boundary pass whose qubits 4 and 5 get RX by lane swaps)
     (15, "c64", "fused", 3), (24, "c64", "fused", 2), (25, "c64", "z2", 3), (25, "c64", "z2", 4),
+    (24, "c64", "fused", 4),
 ]
 
 
