@@ -125,7 +125,7 @@ def test_torch_autograd_function():
     st = State(N, C=10)
     st.run(gr, g, b, "fused")
     e, gg, gb = st.backward(gr, g, b)
-    assert float(E) == pytest.approx(e, rel=1e-13)
+    assert float(E.detach()) == pytest.approx(e, rel=1e-13)
     np.testing.assert_allclose(gt.grad.numpy(), gg, rtol=1e-12, atol=1e-13)
     np.testing.assert_allclose(bt.grad.numpy(), gb, rtol=1e-12, atol=1e-13)
     # one descent step on -<C> increases the expected cut
