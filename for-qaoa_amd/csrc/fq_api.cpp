@@ -798,6 +798,12 @@ static int tile_a_log(int dtype, const std::vector<int> &tile, const std::vector
         if ((g5 + 4) / 5 < (g4 + 3) / 4 && (first || !(l4 && l5)))
             return 5;
     }
+    // the complex128 write-only layer-0 pass as 512 threads x 16 amplitudes
+    // (183 VGPRs, no prefetch to hold): its RX on qubits 5..12 takes one LDS
+    // phase of four register bits instead of two of three; N=30 3.55-3.62 ->
+    // 2.88 ms (5.96 TB/s written), 68.9-69.1 -> 69.5-69.6 layers/s
+    if (dtype == FQ_C128 && Q == 13 && low && first && wide_regs())
+        return 4;
     return slab_a_log(dtype, Q);
 }
 

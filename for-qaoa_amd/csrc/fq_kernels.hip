@@ -219,7 +219,7 @@ __device__ __forceinline__ void mix_natural(T (&v)[1 << A_LOG], const TileArgs &
 // far slower (N=34 plain strided pass 51.2 -> 75.6 ms).
 template <typename R, int A_LOG, int QT, bool COST, bool FIRST, bool GATES, int MIRROR = 0, bool PRE = false,
           int KL = 0, bool LS = false>
-__global__ void __launch_bounds__(GATES ? (A_LOG >= 5 ? 512 : 1024) : 256)
+__global__ void __launch_bounds__(GATES ? (A_LOG >= 5 || (sizeof(R) == 8 && A_LOG >= 4) ? 512 : 1024) : 256)
     k_tile(typename CT<R>::T *__restrict__ sv, const TileArgs a, const double2 *__restrict__ ptab)
 {
     typedef typename CT<R>::T T;
@@ -1374,6 +1374,8 @@ static hipError_t launch_tile_r(int a_log, bool cost, bool first, bool gates, vo
     if constexpr (sizeof(R) == 8) {
         if (a_log == 3 && a.Q == 13)
             return launch_gated<R, 3, 13>(cost, first, sv, a, ptab, stream);
+        if (a_log == 4 && a.Q == 13 && cost && first)
+            return launch_tile_t<R, 4, 13, true, true, true>(sv, a, ptab, stream);
         if (a_log == 3 && a.Q == 12)
             return launch_gated<R, 3, 12>(cost, first, sv, a, ptab, stream);
         switch (a_log) {
